@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""H|psi> throughput on one MI355X: 64-atom Si, 128 bands, FP32-mixed vs FP64.
+
+Workload (synthetic, seeded): Si64 cubic supercell L = 20.52 bohr, Ecut 15 Ha,
+Gamma point, 23,847 plane waves, 72^3 mesh, 256 s+p Kleinman-Bylander
+projectors, 128 bands.  One step = one application of H (kinetic + local
+potential by FFT + non-local projectors) to all 128 bands, with psi, V(r) and
+the projectors already resident in HBM.
+
+``value`` is the FP32-mixed throughput (complex64 FFTs, V(r) product and
+projector GEMMs with FP32 accumulation; psi, H|psi> and the kinetic term in
+FP64).  It is reported only together with the eigenvalue error of the band
+subspace (Rayleigh-Ritz in FP64) against the FP64 path of the same run; the
+FP64 throughput is reported beside it.
+
+Per-kernel durations come from HIP events recorded around each launch on the
+launch stream inside the timed loops (library measurement hook).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H|ψ> bands/sec (64-atom Si, 128 bands): FP32-mixed vs FP64 at ≤1e-6 Ha"
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_work(info, nanl, prec):
+    """Algorithmic bytes (HBM) and flops per band of each kernel.
+
+    s_c, s_r: bytes of a complex / real in the kernel's arithmetic; psi and
+    H|psi> cross the boundary as complex128 (16 B) in both modes."""
+    s_c = 16 if prec == "fp64" else 8
+    s_r = 8 if prec == "fp64" else 4
+    mtxd, ncol = info["mtxd"], info["ncol"]
+    n1, n2, n3 = info["n1"], info["n2"], info["n3"]
+    col = s_c * ncol * n3                 # one pass over the z-column buffer
+    return {
+        "k_col_bwd": dict(bytes=16 * mtxd + col),
+        "k_plane": dict(bytes=2 * col + s_r * n1 * n2 * n3),
+        "k_col_fwd": dict(bytes=col + 16 * mtxd + 16 * mtxd),
+        "k_proj": dict(flops=8 * mtxd * nanl),
+        "k_back": dict(flops=8 * mtxd * nanl),
+    }
+
+
+def roofline_entry(name, work, avg_ms, nb, prec):
+    w = work[name]
+    if "bytes" in w:
+        ach = w["bytes"] * nb / (avg_ms * 1e-3) / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+                "avg_ms": round(avg_ms, 5), "algorithmic_bytes": w["bytes"] * nb}
+    ach = w["flops"] * nb / (avg_ms * 1e-3) / 1e12
+    pk = PEAK_TFLOPS[prec]
+    return {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": pk,
+            "unit": "TFLOP/s", "frac": round(ach / pk, 4), "traffic": None,
+            "avg_ms": round(avg_ms, 5), "algorithmic_flops": w["flops"] * nb}
+
+
+def time_path(torch, hk, plan, psi, hpsi, prec, steps, warmup, kernels):
+    """Times `steps` applies; returns ms/step and per-kernel average ms."""
+    for _ in range(warmup):
+        plan.apply(psi, hpsi, prec)
+    torch.cuda.synchronize()
+    hk.profile_read()
+    hk.profile(True, kernels)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        plan.apply(psi, hpsi, prec)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    hk.profile(False)
+    ms = e0.elapsed_time(e1) / steps
+    per = {k: (v[0] / v[1]) for k, v in hk.profile_read().items() if v[1] > 0}
+    return ms, per
+
+
+def ritz(torch, psi_t, hpsi_t):
+    m = psi_t.conj() @ hpsi_t.T                       # (nb, nb) = psi^H H psi
+    m = 0.5 * (m + m.conj().T)
+    return torch.linalg.eigvalsh(m)
+
+
+def cpu_baseline(d, seconds, nb=8):
+    """The reference hk_psi_c16 (Fortran, NCAR FFT, netlib zgemm) when built,
+    else the C restatement; bounded sample of nb bands per call."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    psi = np.asfortranarray(d["psi"][:, :nb])
+    if O.have_ref():
+        kind, cores = "reference", 1
+        fn = lambda: O.ref_hk_psi(d, psi=psi)
+    else:
+        kind, cores = "port", O.orc_lib().orc_num_threads()
+        fn = lambda: O.orc_hk_psi(d, psi=psi)
+    fn()
+    t0 = time.perf_counter()
+    calls = 0
+    while True:
+        fn()
+        calls += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    out = {"value": round(calls * nb / dt, 2), "unit": "bands/s", "cores": cores, "kind": kind,
+           "sample": f"{calls} calls x {nb} bands of the same 64-atom Si workload ({dt:.1f} s)"}
+    # the C restatement with OpenMP on the host cores, for scale
+    port_threads = O.orc_lib().orc_num_threads()
+    O.orc_hk_psi(d, psi=psi)
+    t0 = time.perf_counter()
+    calls = 0
+    while time.perf_counter() - t0 < seconds / 2:
+        O.orc_hk_psi(d, psi=psi)
+        calls += 1
+    dt = time.perf_counter() - t0
+    out["port_openmp"] = {"value": round(calls * nb / dt, 2), "cores": port_threads}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
+              "H|psi> of independent bands/k-points shards with no communication "
+              "(see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cpw2000_amd import hk_psi as hk
+    from cpw2000_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    log("bench: generating 64-atom Si workload")
+    d = synth.make_config(1)
+    plan = hk.Plan.from_inputs(d)
+    info = plan.info()
+    mtxd, nb, nanl = d["mtxd"], d["neig"], d["nanl"]
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:mtxd].T)).to("cuda")
+    h64 = torch.zeros_like(psi)
+    h32 = torch.zeros_like(psi)
+
+    # accuracy of the FP32-mixed path against FP64 (same inputs, same run)
+    plan.apply(psi, h64, hk.PREC_FP64)
+    plan.apply(psi, h32, hk.PREC_FP32)
+    torch.cuda.synchronize()
+    eig64 = ritz(torch, psi, h64)
+    eig32 = ritz(torch, psi, h32)
+    eig_err = float((eig32 - eig64).abs().max())
+    herr = float(torch.linalg.norm(h32 - h64) / torch.linalg.norm(h64))
+    log(f"bench: FP32-mixed eigenvalue error {eig_err:.2e} Ha, |dH psi|/|H psi| {herr:.2e}")
+
+    # calibration: every kernel's duration (not the timed loops)
+    cal = {}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        _, cal[prec] = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 3, 1, None)
+    work = {p: kernel_work(info, nanl, p) for p in ("fp64", "fp32")}
+
+    def dominant(prec):
+        return max((k for k in cal[prec] if k in work[prec]), key=lambda k: cal[prec][k])
+
+    res = {}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        dk = dominant(prec)
+        log(f"bench: timing {prec} ({args.steps} steps), dominant kernel {dk}")
+        ms, per = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, args.steps,
+                            args.warmup, [dk])
+        res[prec] = dict(ms=ms, bands_per_s=nb / (ms * 1e-3), dominant=dk, dom_ms=per[dk])
+        log(f"bench: {prec}: {ms:.3f} ms/step, {nb / (ms * 1e-3):.0f} bands/s")
+
+    kernels = {p: {k: round(v, 5) for k, v in cal[p].items()} for p in cal}
+    out = {
+        "metric": METRIC,
+        "value": round(res["fp32"]["bands_per_s"], 1),
+        "unit": "bands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["fp32"]["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (FP32 FFT/V/projector GEMMs, FP32 accumulation; FP64 psi, H psi, kinetic)",
+        "data": "synthetic (seeded generator, 64-atom Si)",
+        "config": {"workload": "Si64 cubic L=20.52 bohr, Ecut 15 Ha, Gamma, 128 bands",
+                   "mtxd": mtxd, "mesh": [info["n1"], info["n2"], info["n3"]],
+                   "z_columns": info["ncol"], "nanl": nanl, "bands": nb},
+        "eig_err_ha": eig_err,
+        "eig_tol_ha": 1e-6,
+        "eig_ok": eig_err <= 1e-6,
+        "hpsi_rel_err_fp32": herr,
+        "fp64": {"bands_per_s": round(res["fp64"]["bands_per_s"], 1),
+                 "ms_per_step": round(res["fp64"]["ms"], 4)},
+        "fp32_over_fp64": round(res["fp64"]["ms"] / res["fp32"]["ms"], 3),
+        "roofline": roofline_entry(res["fp32"]["dominant"], work["fp32"], res["fp32"]["dom_ms"],
+                                   nb, "fp32"),
+        "roofline_fp64": roofline_entry(res["fp64"]["dominant"], work["fp64"],
+                                        res["fp64"]["dom_ms"], nb, "fp64"),
+        "kernel_ms": kernels,
+        "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
+                                for k in work[p] if k in cal[p]] for p in cal},
+    }
+    if not args.no_cpu:
+        log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
+        out["cpu_baseline"] = cpu_baseline(d, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,62 @@
+"""Build recipe for the native libraries of cpw2000_amd (in-tree).
+
+  libcpwgpu.so          HIP kernels + C ABI (include/cpwgpu.h), gfx950
+  libcpwgpu_fortran.so  Fortran drop-in hk_psi_c16 over the C ABI (flang)
+
+Run ``python -m cpw2000_amd.build``.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libcpwgpu.so")
+FLIB = os.path.join(PKG, "libcpwgpu_fortran.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang")
+ARCH = os.environ.get("CPWGPU_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+    # no packed FP32 instructions anywhere in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_hip():
+    srcs = [os.path.join(CSRC, "cpwgpu.hip")]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+
+
+def build_fortran():
+    src = os.path.join(PKG, "fortran", "hk_psi_c16.f90")
+    tmp = FLIB + ".tmp"
+    objdir = os.path.join(PKG, "fortran")
+    _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, src,
+          "-L", PKG, "-lcpwgpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FLIB)
+
+
+def build(fortran: bool = True):
+    build_hip()
+    if fortran:
+        build_fortran()
+
+
+if __name__ == "__main__":
+    build(fortran="--no-fortran" not in sys.argv)

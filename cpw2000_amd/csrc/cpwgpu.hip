@@ -1,0 +1,680 @@
+// cpwgpu: MI355X-native H|psi> for cpw2000 -- host side and C ABI.
+//
+// Drop-in for Src/hk_psi/reference/hk_psi_c16.f90 (subroutine hk_psi_c16).
+// The per-k set-up that the reference redoes on every call (index map
+// ipoint, :141-172) is kept in a plan; the band loop (:176-227) becomes three
+// band-batched FFT kernels (fft_local.hpp) and the non-local term
+// (Src/cpw_lib/hk_psi_nl_c16.f90) two MFMA GEMMs (nl_gemm.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cpwgpu.h"
+#include "fft_local.hpp"
+#include "nl_gemm.hpp"
+
+using namespace cpwgpu;
+
+namespace {
+
+thread_local std::string g_err;
+std::mutex g_mu;
+int g_device = 0;
+int g_prec = CPWGPU_PREC_FP64;
+hipStream_t g_stream = nullptr;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(e_ == hipErrorOutOfMemory ? CPWGPU_ERR_NOMEM : CPWGPU_ERR_HIP,     \
+                        "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);  \
+    } while (0)
+
+int ensure_stream() {
+    if (!g_stream) {
+        HIPCHK(hipSetDevice(g_device));
+        HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    }
+    return CPWGPU_OK;
+}
+
+// grow-only device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    int reserve(size_t bytes) {
+        if (bytes <= n) return CPWGPU_OK;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+        n = bytes;
+        return CPWGPU_OK;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Src/ALFA/size_fft.f90:53-76
+void size_fft(const int *km, int *ns, int *mfft, int *mwrk) {
+    static const int inprim[11] = {2, 4, 6, 8, 10, 12, 16, 18, 20, 24, 32};
+    static const int insec[6] = {36, 40, 48, 50, 54, 64};
+    for (int n = 0; n < 3; n++) {
+        int m = 2 * km[n] + 1;
+        if (m < 32) {
+            for (int j = 0; j < 11; j++) {
+                ns[n] = inprim[j];
+                if (m <= ns[n]) break;
+            }
+        } else {
+            int k = (int)(logf((float)m) / logf(2.0f));
+            for (int j = 0; j < 6; j++) {
+                ns[n] = (1 << (k - 5)) * insec[j];
+                if (m <= ns[n]) break;
+            }
+        }
+    }
+    *mfft = (ns[0] + 1) * ns[1] * ns[2];
+    *mwrk = 4 * std::max({ns[0] * ns[1], ns[0] * ns[2], ns[1] * ns[2]});
+}
+
+// radices 4..4, 2, 3.., 5..; false if n has another prime factor
+bool radices(int n, int *rad, int *nrad) {
+    int k = 0;
+    while (n % 4 == 0 && k < MAXRAD) { rad[k++] = 4; n /= 4; }
+    while (n % 2 == 0 && k < MAXRAD) { rad[k++] = 2; n /= 2; }
+    while (n % 3 == 0 && k < MAXRAD) { rad[k++] = 3; n /= 3; }
+    while (n % 5 == 0 && k < MAXRAD) { rad[k++] = 5; n /= 5; }
+    *nrad = k;
+    return n == 1;
+}
+
+int min_radix(const int *rad, int nrad) {
+    int r = 5;
+    for (int s = 0; s < nrad; s++) r = std::min(r, rad[s]);
+    return r;
+}
+
+// ---- per-kernel event timing (measurement hook) ----------------------------
+struct Prof {
+    bool on = false;
+    unsigned mask = 0;
+    struct Rec { int k; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+} g_prof;
+
+struct ProfScope {
+    int k;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(int k_, hipStream_t s_) : k(k_), s(s_) {
+        if (g_prof.on && (g_prof.mask >> k & 1u)) {
+            a = g_prof.get();
+            if (a && hipEventRecord(a, s) != hipSuccess) a = nullptr;
+        }
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = g_prof.get();
+        if (b && hipEventRecord(b, s) == hipSuccess) g_prof.recs.push_back({k, a, b});
+    }
+};
+
+constexpr int COL_CT = 16;   // z-columns per block of the column kernels
+constexpr int KMAX = 8;      // butterflies per thread per FFT stage
+
+}  // namespace
+
+struct cpwgpu_plan_s {
+    FftGeom g{};
+    int mtxd = 0, ng = 0, nanl = 0;
+    int kdmax[3] = {0, 0, 0};
+    double inv_n = 1.0;
+    size_t ntot = 0;          // id*n2*n3
+    int mwrk = 0;
+    // host copies used to recognise an unchanged k-point in the drop-in
+    std::vector<long> key_pts;
+    std::vector<double> h_ekpg;
+    // device arrays
+    DevBuf col_ptr, col_uy, ent_i, ent_cz, ekpg, tw[3];
+    DevBuf v64, v32;          // V(r) in FP64 and FP32
+    DevBuf anl64, anl32, xnl; // projectors
+    long lda = 0;             // leading dimension of anl64/anl32
+    bool anl32_valid = false;
+    // workspaces
+    DevBuf colbuf, part, dhd;
+    ~cpwgpu_plan_s() {
+        col_ptr.release(); col_uy.release(); ent_i.release(); ent_cz.release(); ekpg.release();
+        for (auto &t : tw) t.release();
+        v64.release(); v32.release(); anl64.release(); anl32.release(); xnl.release();
+        colbuf.release(); part.release(); dhd.release();
+    }
+};
+
+namespace {
+
+// lines of the G list of each plane wave, wrapped into the mesh
+// (Src/hk_psi/reference/hk_psi_c16.f90:143-158)
+int plane_wave_points(int mtxd, int ng, const int *kgv, const int *isort, const int *ns,
+                      std::vector<int> &kx, std::vector<int> &ky, std::vector<int> &kz, int *kd) {
+    kd[0] = kd[1] = kd[2] = 0;
+    kx.resize(mtxd); ky.resize(mtxd); kz.resize(mtxd);
+    for (int i = 0; i < mtxd; i++) {
+        const int it = isort[i] - 1;
+        if (it < 0 || it >= ng) return fail(CPWGPU_ERR_ARG, "isort(%d) = %d outside 1..ng=%d", i + 1, it + 1, ng);
+        int k[3];
+        for (int j = 0; j < 3; j++) {
+            k[j] = kgv[3L * it + j];
+            kd[j] = std::max(kd[j], std::abs(k[j]));
+            if (k[j] < 0) k[j] += ns[j];
+        }
+        kx[i] = k[0]; ky[i] = k[1]; kz[i] = k[2];
+    }
+    return CPWGPU_OK;
+}
+
+template <typename T> int upload(DevBuf &b, const T *h, size_t n) {
+    int st = b.reserve(sizeof(T) * std::max<size_t>(n, 1));
+    if (st) return st;
+    if (n) HIPCHK(hipMemcpyAsync(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice, g_stream));
+    return CPWGPU_OK;
+}
+
+int build_plan(cpwgpu_plan_s *p, int mtxd, int ng, const int *kgv, const double *ekpg,
+               const int *isort, const int *kmscr, int mxddim, int mxdgve) {
+    if (ng > mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", ng, mxdgve);
+    if (mtxd < 0 || mtxd > mxddim) return fail(CPWGPU_ERR_MTXD, "mtxd = %d, mxddim = %d", mtxd, mxddim);
+    int ns[3] = {kmscr[3], kmscr[4], kmscr[5]};
+    const int id = kmscr[6];
+    for (int j = 0; j < 3; j++)
+        if (ns[j] < 2 || ns[j] > 4095)
+            return fail(CPWGPU_ERR_SIZE, "FFT mesh %d %d %d outside 2..4095", ns[0], ns[1], ns[2]);
+    if (id < ns[0]) return fail(CPWGPU_ERR_SIZE, "kmscr(7) = %d < n1 = %d", id, ns[0]);
+    FftGeom &g = p->g;
+    for (int j = 0; j < 3; j++) {
+        g.n[j] = ns[j];
+        if (!radices(ns[j], g.rad[j], &g.nrad[j]))
+            return fail(CPWGPU_ERR_SIZE, "FFT length %d is not of the form 2^a 3^b 5^c", ns[j]);
+    }
+    g.id = id;
+    int nsz[3], mfft;
+    size_fft(kmscr, nsz, &mfft, &p->mwrk);
+    std::vector<int> kx, ky, kz;
+    int kd[3];
+    int st = plane_wave_points(mtxd, ng, kgv, isort, ns, kx, ky, kz, kd);
+    if (st) return st;
+    // no wrap-around (Src/hk_psi/reference/hk_psi_c16.f90:162-172)
+    if (kd[0] > (ns[0] - 1) / 2 || kd[1] > (ns[1] - 1) / 2 || kd[2] > (ns[2] - 1) / 2)
+        return fail(CPWGPU_ERR_WRAP, "dimension of k index %d %d %d exceeds %d %d %d", kd[0], kd[1],
+                    kd[2], (ns[0] - 1) / 2, (ns[1] - 1) / 2, (ns[2] - 1) / 2);
+    for (int j = 0; j < 3; j++) { g.kd[j] = kd[j]; p->kdmax[j] = kd[j]; }
+    g.nx = 2 * kd[0] + 1;
+    g.mtxd = mtxd;
+    p->mtxd = mtxd;
+    p->ng = ng;
+    p->ntot = (size_t)id * ns[1] * ns[2];
+    p->inv_n = 1.0 / ((double)ns[0] * ns[1] * ns[2]);
+
+    // columns = distinct (x, y), numbered in (y, x) order; CSR of plane waves
+    const long nxy = (long)ns[0] * ns[1];
+    std::vector<int> colid(nxy, -1), count;
+    for (int i = 0; i < mtxd; i++) colid[(long)ky[i] * ns[0] + kx[i]] = 0;
+    int ncol = 0;
+    std::vector<int> uy;
+    for (long q = 0; q < nxy; q++)
+        if (colid[q] == 0) {
+            colid[q] = ncol++;
+            const int x = (int)(q % ns[0]), y = (int)(q / ns[0]);
+            const int u = x <= kd[0] ? x : x - (ns[0] - g.nx);
+            uy.push_back((y << 16) | u);
+        }
+    if (ncol >= (1 << 19)) return fail(CPWGPU_ERR_SIZE, "too many z-columns (%d)", ncol);
+    g.ncol = ncol;
+    g.ncolp = ((std::max(ncol, 1) + 31) / 32) * 32;
+    std::vector<int> cptr(ncol + 1, 0), ei(mtxd), ecz(mtxd);
+    for (int i = 0; i < mtxd; i++) cptr[colid[(long)ky[i] * ns[0] + kx[i]] + 1]++;
+    for (int c = 0; c < ncol; c++) cptr[c + 1] += cptr[c];
+    std::vector<int> fillp(cptr.begin(), cptr.end() - 1);
+    for (int i = 0; i < mtxd; i++) {
+        const int c = colid[(long)ky[i] * ns[0] + kx[i]];
+        const int e = fillp[c]++;
+        ei[e] = i;
+        ecz[e] = (c << 12) | kz[i];
+    }
+    p->key_pts.resize(mtxd);
+    for (int i = 0; i < mtxd; i++) p->key_pts[i] = ((long)kz[i] * ns[1] + ky[i]) * id + kx[i];
+
+    if ((st = ensure_stream())) return st;
+    if ((st = upload(p->col_ptr, cptr.data(), cptr.size()))) return st;
+    if ((st = upload(p->col_uy, uy.data(), uy.size()))) return st;
+    if ((st = upload(p->ent_i, ei.data(), ei.size()))) return st;
+    if ((st = upload(p->ent_cz, ecz.data(), ecz.size()))) return st;
+    p->h_ekpg.assign(ekpg, ekpg + mtxd);
+    if ((st = upload(p->ekpg, ekpg, mtxd))) return st;
+    for (int j = 0; j < 3; j++) {
+        std::vector<double2> t(ns[j]);
+        for (int k = 0; k < ns[j]; k++) {
+            long double a = 2.0L * 3.141592653589793238462643383279502884L * k / ns[j];
+            t[k] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        if ((st = upload(p->tw[j], t.data(), t.size()))) return st;
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+// LDS bytes of the kernels for element size cs
+size_t lds_col(const FftGeom &g, size_t cs) { return cs * (g.n[2] + (size_t)COL_CT * (g.n[2] + 1)); }
+int plane_rows(const FftGeom &g, size_t cs, size_t *bytes) {
+    const int nxp = g.nx | 1;
+    const size_t fixed = cs * ((size_t)g.n[0] + g.n[1] + (size_t)g.n[1] * nxp);
+    const size_t budget = 80 * 1024;  // two blocks per CU when it fits
+    int rb = std::min(g.n[1], 32);
+    while (rb > 1 && fixed + cs * (size_t)rb * (g.n[0] + 1) > budget) rb--;
+    *bytes = fixed + cs * (size_t)rb * (g.n[0] + 1);
+    return rb;
+}
+
+template <typename R>
+int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+            hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t cs = sizeof(cplx<R>);
+    const R *vr;
+    if (sizeof(R) == 8) {
+        vr = (const R *)p->v64.p;
+    } else {
+        vr = (const R *)p->v32.p;
+    }
+    if (!vr) return fail(CPWGPU_ERR_ARG, "potential not set");
+    // bands per chunk so that colbuf stays below 4 GiB
+    const size_t per_band = cs * (size_t)g.n[2] * g.ncolp;
+    const int nbc = (int)std::max<size_t>(1, std::min<size_t>(neig, (4ull << 30) / per_band));
+    int st = p->colbuf.reserve(per_band * nbc);
+    if (st) return st;
+    cplx<R> *colbuf = (cplx<R> *)p->colbuf.p;
+    const int ntiles = (g.ncol + COL_CT - 1) / COL_CT;
+    const size_t lcol = lds_col(g, cs);
+    size_t lplane;
+    const int rb = plane_rows(g, cs, &lplane);
+    if (lplane > 160 * 1024 || lcol > 160 * 1024)
+        return fail(CPWGPU_ERR_SIZE, "FFT mesh too large for LDS (%zu / %zu bytes)", lplane, lcol);
+    static bool attr_done[2] = {false, false};
+    if (!attr_done[sizeof(R) == 8]) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_col_bwd<R, COL_CT, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void *)k_col_fwd<R, COL_CT, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void *)k_plane<R, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_done[sizeof(R) == 8] = true;
+    }
+    const double2 *tw1 = (const double2 *)p->tw[0].p, *tw2 = (const double2 *)p->tw[1].p,
+                  *tw3 = (const double2 *)p->tw[2].p;
+    for (int b0 = 0; b0 < neig; b0 += nbc) {
+        const int nb = std::min(nbc, neig - b0);
+        const double2 *pp = (const double2 *)psi + (long)b0 * ldp;
+        double2 *hp = (double2 *)hpsi + (long)b0 * ldh;
+        if (g.ncol > 0) {
+            { ProfScope ps_(CPWGPU_K_COL_BWD, s);
+            hipLaunchKernelGGL((k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * nb), dim3(COL_THREADS), lcol, s,
+                               pp, ldp, colbuf, (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
+                               (const int *)p->ent_cz.p, tw3, g, ntiles); }
+            { ProfScope ps_(CPWGPU_K_PLANE, s);
+            hipLaunchKernelGGL((k_plane<R, KMAX>), dim3(g.n[2] * nb), dim3(PLANE_THREADS), lplane, s,
+                               colbuf, (const int *)p->col_uy.p, vr, tw1, tw2, g, rb); }
+            { ProfScope ps_(CPWGPU_K_COL_FWD, s);
+            hipLaunchKernelGGL((k_col_fwd<R, COL_CT, KMAX>), dim3(ntiles * nb), dim3(COL_THREADS), lcol, s,
+                               pp, ldp, hp, ldh, (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
+                               (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
+                               (const double *)p->ekpg.p, tw3, g, ntiles, p->inv_n); }
+            HIPCHK(hipGetLastError());
+        }
+    }
+    // non-local term
+    const int nanl = p->nanl;
+    if (nanl > 0 && g.mtxd > 0) {
+        const cplx<R> *anl;
+        if (sizeof(R) == 8) {
+            anl = (const cplx<R> *)p->anl64.p;
+        } else {
+            if (!p->anl32_valid) {
+                if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda * nanl))) return st;
+                const long tot = (long)g.mtxd * nanl;
+                hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                                   (const double2 *)p->anl64.p, p->lda, (cplx<float> *)p->anl32.p,
+                                   p->lda, g.mtxd, nanl);
+                HIPCHK(hipGetLastError());
+                p->anl32_valid = true;
+            }
+            anl = (const cplx<R> *)p->anl32.p;
+        }
+        const int tm = (nanl + PJ_BM - 1) / PJ_BM, tn = (neig + PJ_BN - 1) / PJ_BN;
+        const int ksteps = (g.mtxd + PJ_BK - 1) / PJ_BK;
+        int nsl = std::max(1, std::min(ksteps, (1024 + tm * tn - 1) / (tm * tn)));
+        const int kslice = ((ksteps + nsl - 1) / nsl) * PJ_BK;
+        nsl = (g.mtxd + kslice - 1) / kslice;
+        if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
+        if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
+        { ProfScope ps_(CPWGPU_K_PROJ, s);
+        hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, p->lda,
+                           (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig, kslice); }
+        const long tot = (long)nanl * neig;
+        { ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
+        hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                           (const cplx<R> *)p->part.p, (const double *)p->xnl.p, (cplx<R> *)p->dhd.p,
+                           nanl, neig, nsl); }
+        { ProfScope ps_(CPWGPU_K_BACK, s);
+        hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+                           dim3(GEMM_THREADS), 0, s, anl, p->lda, (const cplx<R> *)p->dhd.p,
+                           (double2 *)hpsi, ldh, g.mtxd, nanl, neig); }
+        HIPCHK(hipGetLastError());
+    }
+    return CPWGPU_OK;
+}
+
+int set_potential(cpwgpu_plan_s *p, const double *vscr, int mxdscr) {
+    if (p->mwrk > mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", p->mwrk, mxdscr);
+    if (p->ntot > (size_t)mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mesh id*n2*n3 = %zu is greater than mxdscr = %d", p->ntot, mxdscr);
+    int st = ensure_stream();
+    if (st) return st;
+    if ((st = upload(p->v64, vscr, p->ntot))) return st;
+    if ((st = p->v32.reserve(sizeof(float) * p->ntot))) return st;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((p->ntot + 255) / 256)), dim3(256), 0, g_stream,
+                       (const double *)p->v64.p, (float *)p->v32.p, (long)p->ntot);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+int set_projectors_host(cpwgpu_plan_s *p, const double *anlga, const double *xnlkb, int nanl,
+                        int mxddim) {
+    if (nanl < 0) return fail(CPWGPU_ERR_ARG, "nanl = %d", nanl);
+    int st = ensure_stream();
+    if (st) return st;
+    p->nanl = nanl;
+    p->anl32_valid = false;
+    if (nanl == 0) return CPWGPU_OK;
+    if (!anlga || !xnlkb) return fail(CPWGPU_ERR_ARG, "null projector array");
+    p->lda = std::max(1, p->mtxd);
+    if ((st = p->anl64.reserve(16 * (size_t)p->lda * nanl))) return st;
+    if (p->mtxd > 0)
+        HIPCHK(hipMemcpy2DAsync(p->anl64.p, 16 * (size_t)p->lda, anlga, 16 * (size_t)mxddim,
+                                16 * (size_t)p->mtxd, nanl, hipMemcpyHostToDevice, g_stream));
+    if ((st = upload(p->xnl, xnlkb, nanl))) return st;
+    return CPWGPU_OK;
+}
+
+// ---- drop-in state --------------------------------------------------------
+struct DropIn {
+    cpwgpu_plan_s *plan = nullptr;
+    int key_ng = -1, key_mtxd = -1;
+    int key_kmscr[7] = {0};
+    DevBuf psi, hpsi;
+} g_drop;
+
+int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
+                 const int *kmscr, int mxddim, int mxdgve, cpwgpu_plan_s **out) {
+    cpwgpu_plan_s *p = g_drop.plan;
+    bool same = p && g_drop.key_mtxd == mtxd && g_drop.key_ng == ng &&
+                std::equal(kmscr, kmscr + 7, g_drop.key_kmscr);
+    if (same) {
+        // same plane waves in the same order on the same mesh?
+        std::vector<int> kx, ky, kz;
+        int kd[3];
+        int ns[3] = {kmscr[3], kmscr[4], kmscr[5]};
+        if (ng > mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", ng, mxdgve);
+        if (mtxd > mxddim) return fail(CPWGPU_ERR_MTXD, "mtxd = %d, mxddim = %d", mtxd, mxddim);
+        int st = plane_wave_points(mtxd, ng, kgv, isort, ns, kx, ky, kz, kd);
+        if (st) return st;
+        for (int i = 0; i < mtxd && same; i++)
+            same = p->key_pts[i] == ((long)kz[i] * ns[1] + ky[i]) * kmscr[6] + kx[i];
+    }
+    if (!same) {
+        delete g_drop.plan;
+        g_drop.plan = nullptr;
+        p = new cpwgpu_plan_s();
+        int st = build_plan(p, mtxd, ng, kgv, ekpg, isort, kmscr, mxddim, mxdgve);
+        if (st) {
+            delete p;
+            return st;
+        }
+        g_drop.plan = p;
+        g_drop.key_mtxd = mtxd;
+        g_drop.key_ng = ng;
+        std::copy(kmscr, kmscr + 7, g_drop.key_kmscr);
+    } else if (!std::equal(ekpg, ekpg + mtxd, p->h_ekpg.begin())) {
+        p->h_ekpg.assign(ekpg, ekpg + mtxd);
+        int st = upload(p->ekpg, ekpg, mtxd);
+        if (st) return st;
+    }
+    *out = p;
+    return CPWGPU_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int cpwgpu_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(CPWGPU_ERR_ARG, "device %d of %d", device, n);
+    if (g_stream && device != g_device) {
+        HIPCHK(hipStreamDestroy(g_stream));
+        g_stream = nullptr;
+    }
+    g_device = device;
+    return ensure_stream();
+}
+
+int cpwgpu_finalize(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    delete g_drop.plan;
+    g_drop.plan = nullptr;
+    g_drop.key_mtxd = -1;
+    g_drop.psi.release();
+    g_drop.hpsi.release();
+    if (g_stream) {
+        HIPCHK(hipStreamSynchronize(g_stream));
+        HIPCHK(hipStreamDestroy(g_stream));
+        g_stream = nullptr;
+    }
+    return CPWGPU_OK;
+}
+
+const char *cpwgpu_last_error(void) { return g_err.c_str(); }
+
+int cpwgpu_set_precision(int prec) {
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32)
+        return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    g_prec = prec;
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_create(cpwgpu_plan_t *plan, int mtxd, int ng, const int *kgv, const double *ekpg,
+                       const int *isort, const int *kmscr, int mxddim, int mxdgve) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || !kmscr || (mtxd > 0 && (!kgv || !ekpg || !isort)))
+        return fail(CPWGPU_ERR_ARG, "null argument");
+    *plan = nullptr;
+    cpwgpu_plan_s *p = new cpwgpu_plan_s();
+    int st = build_plan(p, mtxd, ng, kgv, ekpg, isort, kmscr, mxddim, mxdgve);
+    if (st) {
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_destroy(cpwgpu_plan_t plan) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_stream) hipStreamSynchronize(g_stream);
+    hipDeviceSynchronize();
+    delete plan;
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_potential(cpwgpu_plan_t plan, const double *vscr, int mxdscr) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || !vscr) return fail(CPWGPU_ERR_ARG, "null argument");
+    int st = set_potential(plan, vscr, mxdscr);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_projectors(cpwgpu_plan_t plan, const double *anlga, const double *xnlkb,
+                               int nanl, int mxddim) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (mxddim < plan->mtxd) return fail(CPWGPU_ERR_MTXD, "mxddim %d < mtxd %d", mxddim, plan->mtxd);
+    int st = set_projectors_host(plan, anlga, xnlkb, nanl, mxddim);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_dev, long lda,
+                                      const double *xnlkb_dev, int nanl) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || nanl < 0) return fail(CPWGPU_ERR_ARG, "bad argument");
+    int st = ensure_stream();
+    if (st) return st;
+    plan->nanl = nanl;
+    plan->anl32_valid = false;
+    if (nanl == 0) return CPWGPU_OK;
+    if (!anlga_dev || !xnlkb_dev || lda < plan->mtxd) return fail(CPWGPU_ERR_ARG, "bad projector arrays");
+    plan->lda = std::max(1, plan->mtxd);
+    if ((st = plan->anl64.reserve(16 * (size_t)plan->lda * nanl))) return st;
+    if ((st = plan->xnl.reserve(8 * (size_t)nanl))) return st;
+    if (plan->mtxd > 0)
+        HIPCHK(hipMemcpy2DAsync(plan->anl64.p, 16 * (size_t)plan->lda, anlga_dev, 16 * (size_t)lda,
+                                16 * (size_t)plan->mtxd, nanl, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(plan->xnl.p, xnlkb_dev, 8 * (size_t)nanl, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                      double *hpsi_dev, long ldh, int prec, void *stream) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
+    if (neig == 0) return CPWGPU_OK;
+    if (!psi_dev || !hpsi_dev) return fail(CPWGPU_ERR_ARG, "null psi/hpsi");
+    if (ldp < plan->mtxd || ldh < plan->mtxd) return fail(CPWGPU_ERR_ARG, "leading dimension < mtxd");
+    int st = ensure_stream();
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;  // NULL: the null stream, as in HIP
+    if (prec == CPWGPU_PREC_FP64) return apply_t<double>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s);
+    if (prec == CPWGPU_PREC_FP32) return apply_t<float>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s);
+    return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+}
+
+int cpwgpu_profile(int on, unsigned mask) {
+    g_prof.on = on != 0;
+    g_prof.mask = mask;
+    return CPWGPU_OK;
+}
+
+int cpwgpu_profile_read(double *ms, long *launches, int n) {
+    if (n < 0 || (n > 0 && (!ms || !launches))) return fail(CPWGPU_ERR_ARG, "bad profile buffers");
+    for (int k = 0; k < n; k++) { ms[k] = 0.0; launches[k] = 0; }
+    for (auto &r : g_prof.recs) {
+        HIPCHK(hipEventSynchronize(r.b));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        if (r.k < n) { ms[r.k] += t; launches[r.k] += 1; }
+        g_prof.pool.push_back(r.a);
+        g_prof.pool.push_back(r.b);
+    }
+    g_prof.recs.clear();
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out) {
+    if (!plan || !out) return fail(CPWGPU_ERR_ARG, "null argument");
+    const FftGeom &g = plan->g;
+    int v[10] = {plan->mtxd, g.n[0], g.n[1], g.n[2], g.id, g.kd[0], g.kd[1], g.kd[2], g.ncol, plan->nanl};
+    std::copy(v, v + 10, out);
+    return CPWGPU_OK;
+}
+
+int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, double *hpsi,
+                      const int *lnewanl, const int *ng, const int *kgv, const double *ekpg,
+                      const int *isort, const double *vscr, const int *kmscr, const double *anlga,
+                      const double *xnlkb, const int *nanl, const int *mxddim, const int *mxdbnd,
+                      const int *mxdanl, const int *mxdgve, const int *mxdscr) {
+    (void)lnewanl;
+    (void)mxdbnd;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!mtxd || !neig || !ng || !kmscr || !nanl || !mxddim || !mxdanl || !mxdgve || !mxdscr)
+        return fail(CPWGPU_ERR_ARG, "null scalar argument");
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", *ng, *mxdgve);
+    int ns[3], mfft, mwrk;
+    size_fft(kmscr, ns, &mfft, &mwrk);
+    if (mwrk > *mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", mwrk, *mxdscr);
+    if (*nanl > *mxdanl) return fail(CPWGPU_ERR_ARG, "nanl = %d > mxdanl = %d", *nanl, *mxdanl);
+    if (*neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", *neig);
+    cpwgpu_plan_s *p = nullptr;
+    int st = drop_in_plan(*mtxd, *ng, kgv, ekpg, isort, kmscr, *mxddim, *mxdgve, &p);
+    if (st) return st;
+    if (*neig == 0 || *mtxd == 0) return CPWGPU_OK;
+    if (!psi || !hpsi || !vscr) return fail(CPWGPU_ERR_ARG, "null array argument");
+    if ((st = set_potential(p, vscr, *mxdscr))) return st;
+    // projectors are re-sent on every call: the reference callers never clear lnewanl
+    if ((st = set_projectors_host(p, anlga, xnlkb, *nanl, *mxddim))) return st;
+    const size_t colb = 16 * (size_t)(*mtxd);
+    if ((st = g_drop.psi.reserve(colb * *neig))) return st;
+    if ((st = g_drop.hpsi.reserve(colb * *neig))) return st;
+    HIPCHK(hipMemcpy2DAsync(g_drop.psi.p, colb, psi, 16 * (size_t)*mxddim, colb, *neig,
+                            hipMemcpyHostToDevice, g_stream));
+    const double *dpsi = (const double *)g_drop.psi.p;
+    double *dhpsi = (double *)g_drop.hpsi.p;
+    st = g_prec == CPWGPU_PREC_FP32 ? apply_t<float>(p, *neig, dpsi, *mtxd, dhpsi, *mtxd, g_stream)
+                                    : apply_t<double>(p, *neig, dpsi, *mtxd, dhpsi, *mtxd, g_stream);
+    if (st) return st;
+    HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)*mxddim, g_drop.hpsi.p, colb, colb, *neig,
+                            hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+"""Python mirror of the cpw2000 ``hk_psi_c16`` interface over the C ABI.
+
+``hk_psi_c16`` takes the Fortran subroutine's arguments, in its order and
+meaning (Src/hk_psi/reference/hk_psi_c16.f90:28-78), as numpy arrays in
+Fortran layout, and fills ``hpsi`` in place.  Where the reference stops the
+program (ng > mxdgve, FFT work size > mxdscr, sphere wrap-around) this raises
+``CpwGpuError`` with the library's status code.
+
+``Plan`` is the device-resident path: index maps, V(r) and projectors are set
+once per k-point and H is applied to torch tensors already in HBM.
+
+The library is loaded from the package directory; there is no fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcpwgpu.so")
+FLIB_PATH = os.path.join(PKG, "libcpwgpu_fortran.so")
+
+OK = 0
+ERR_NG = 1
+ERR_MXDSCR = 2
+ERR_WRAP = 3
+ERR_MTXD = 4
+ERR_MODE = 5
+ERR_SIZE = 6
+ERR_ARG = 7
+ERR_HIP = 10
+ERR_NOMEM = 11
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+# every symbol declared in include/cpwgpu.h
+EXPORTS = (
+    "cpwgpu_init", "cpwgpu_finalize", "cpwgpu_last_error", "cpwgpu_set_precision",
+    "cpwgpu_hk_psi_c16", "cpwgpu_plan_create", "cpwgpu_plan_destroy",
+    "cpwgpu_plan_set_potential", "cpwgpu_plan_set_projectors",
+    "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
+    "cpwgpu_profile", "cpwgpu_profile_read",
+)
+
+# kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
+KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
+
+
+class CpwGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cpwgpu status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libcpwgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        # PyTorch bundles its own libamdhip64.so.7 (same soname as the system
+        # one this library links).  Loading torch first makes the whole
+        # process use a single HIP runtime, shared with torch tensors/streams.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cpw2000_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.c_void_p
+        L.cpwgpu_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            if name != "cpwgpu_last_error":
+                getattr(L, name).restype = C.c_int
+        L.cpwgpu_plan_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int]
+        L.cpwgpu_plan_destroy.argtypes = [vp]
+        L.cpwgpu_plan_set_potential.argtypes = [vp, dp, C.c_int]
+        L.cpwgpu_plan_set_projectors.argtypes = [vp, dp, dp, C.c_int, C.c_int]
+        L.cpwgpu_plan_set_projectors_device.argtypes = [vp, dp, C.c_long, dp, C.c_int]
+        L.cpwgpu_plan_apply.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
+        L.cpwgpu_plan_info.argtypes = [vp, ip]
+        L.cpwgpu_profile.argtypes = [C.c_int, C.c_uint]
+        L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != OK:
+        raise CpwGpuError(st, lib().cpwgpu_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _iref(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def set_precision(prec):
+    _check(lib().cpwgpu_set_precision(int(prec)))
+
+
+def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr, anlga, xnlkb,
+               nanl, mxddim, mxdbnd, mxdanl, mxdgve, mxdscr):
+    """H|psi> with the arguments of Src/hk_psi/reference/hk_psi_c16.f90:28-32.
+
+    Arrays are numpy, Fortran layout: psi/hpsi complex128 (mxddim, >=neig),
+    kgv int32 (3, mxdgve), anlga complex128 (mxddim, mxdanl).  hpsi is
+    written in place (rows 0..mtxd-1 of the first neig columns)."""
+    for name, arr, dt in (("psi", psi, np.complex128), ("hpsi", hpsi, np.complex128),
+                          ("kgv", kgv, np.int32), ("ekpg", ekpg, np.float64),
+                          ("isort", isort, np.int32), ("vscr", vscr, np.float64),
+                          ("kmscr", kmscr, np.int32), ("anlga", anlga, np.complex128),
+                          ("xnlkb", xnlkb, np.float64)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    lnew = C.c_int(1 if lnewanl else 0)
+    st = lib().cpwgpu_hk_psi_c16(
+        _iref(mtxd), _iref(neig), _p(psi), _p(hpsi), C.byref(lnew), _iref(ng), _p(kgv), _p(ekpg),
+        _p(isort), _p(vscr), _p(kmscr), _p(anlga), _p(xnlkb), _iref(nanl), _iref(mxddim),
+        _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr))
+    _check(st)
+    return hpsi
+
+
+def hk_psi_c16_dict(d, psi=None, hpsi=None, neig=None):
+    """hk_psi_c16 on a dictionary of inputs (keys named as the Fortran dummies)."""
+    psi = np.asfortranarray(d["psi"] if psi is None else psi, np.complex128)
+    neig = psi.shape[1] if neig is None else neig
+    if hpsi is None:
+        hpsi = np.zeros_like(psi, order="F")
+    return hk_psi_c16(d["mtxd"], neig, psi, hpsi, True, d["ng"],
+                      np.asfortranarray(d["kgv"], np.int32), np.asfortranarray(d["ekpg"]),
+                      np.asfortranarray(d["isort"], np.int32), np.asfortranarray(d["vscr"]),
+                      np.asfortranarray(d["kmscr"], np.int32),
+                      np.asfortranarray(d["anlga"], np.complex128), np.asfortranarray(d["xnlkb"]),
+                      d["nanl"], d["mxddim"], max(psi.shape[1], 1), max(d["mxdanl"], 1),
+                      d["mxdgve"], d["mxdscr"])
+
+
+def profile(on, kernels=None):
+    """Per-kernel HIP-event timing of later applies (kernels: names of KERNELS)."""
+    mask = 0
+    for k in (KERNELS if kernels is None else kernels):
+        mask |= 1 << KERNELS.index(k)
+    _check(lib().cpwgpu_profile(1 if on else 0, mask))
+
+
+def profile_read():
+    """{kernel: (summed ms, launches)} since the previous read."""
+    n = len(KERNELS)
+    ms = (C.c_double * n)()
+    cnt = (C.c_long * n)()
+    _check(lib().cpwgpu_profile_read(ms, cnt, n))
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS)}
+
+
+class Plan:
+    """One k-point resident on the device (see cpwgpu_plan_* in include/cpwgpu.h)."""
+
+    def __init__(self, mtxd, ng, kgv, ekpg, isort, kmscr, mxddim=None, mxdgve=None):
+        self._h = C.c_void_p()
+        kgv = np.asfortranarray(kgv, np.int32)
+        ekpg = np.ascontiguousarray(ekpg, np.float64)
+        isort = np.ascontiguousarray(isort, np.int32)
+        kmscr = np.ascontiguousarray(kmscr, np.int32)
+        mxddim = mtxd if mxddim is None else mxddim
+        mxdgve = ng if mxdgve is None else mxdgve
+        _check(lib().cpwgpu_plan_create(C.byref(self._h), int(mtxd), int(ng), _p(kgv), _p(ekpg),
+                                        _p(isort), _p(kmscr), int(mxddim), int(mxdgve)))
+        self.mtxd = int(mtxd)
+
+    @classmethod
+    def from_inputs(cls, d, projectors=True):
+        p = cls(d["mtxd"], d["ng"], d["kgv"], d["ekpg"], d["isort"], d["kmscr"], d["mxddim"],
+                d["mxdgve"])
+        p.set_potential(d["vscr"])
+        if projectors:
+            p.set_projectors(d["anlga"], d["xnlkb"], d["nanl"])
+        return p
+
+    def info(self):
+        out = (C.c_int * 10)()
+        _check(lib().cpwgpu_plan_info(self._h, out))
+        keys = ("mtxd", "n1", "n2", "n3", "id", "kd1", "kd2", "kd3", "ncol", "nanl")
+        return dict(zip(keys, list(out)))
+
+    def set_potential(self, vscr):
+        v = np.ascontiguousarray(vscr, np.float64)
+        _check(lib().cpwgpu_plan_set_potential(self._h, _p(v), int(v.shape[0])))
+
+    def set_projectors(self, anlga, xnlkb, nanl=None):
+        a = np.asfortranarray(anlga, np.complex128)
+        x = np.ascontiguousarray(xnlkb, np.float64)
+        nanl = a.shape[1] if nanl is None else nanl
+        _check(lib().cpwgpu_plan_set_projectors(self._h, _p(a), _p(x), int(nanl), int(a.shape[0])))
+
+    def set_projectors_device(self, anlga_t, xnlkb_t):
+        """anlga_t: complex128 device tensor (nanl, lda) (each projector contiguous)."""
+        nanl, lda = anlga_t.shape
+        _check(lib().cpwgpu_plan_set_projectors_device(
+            self._h, C.c_void_p(anlga_t.data_ptr()), int(anlga_t.stride(0)),
+            C.c_void_p(xnlkb_t.data_ptr()), int(nanl)))
+
+    def apply(self, psi_t, hpsi_t, prec=PREC_FP64, stream=None):
+        """H|psi> on complex128 device tensors shaped (neig, ld) (band-major,
+        each band contiguous: the column-major psi(ld, neig) of the reference)."""
+        import torch
+
+        neig = psi_t.shape[0]
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        _check(lib().cpwgpu_plan_apply(self._h, int(neig), C.c_void_p(psi_t.data_ptr()),
+                                       int(psi_t.stride(0)), C.c_void_p(hpsi_t.data_ptr()),
+                                       int(hpsi_t.stride(0)), int(prec), C.c_void_p(stream)))
+        return hpsi_t
+
+    def close(self):
+        if self._h:
+            lib().cpwgpu_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,122 @@
+/*
+ * cpwgpu -- MI355X-native H|psi> for cpw2000.
+ *
+ * C ABI of the shared library libcpwgpu.so.  Plain pointers and sizes only.
+ * Complex arrays are interleaved (re, im) pairs: Fortran complex(REAL64)
+ * (16 bytes) for the double-precision arrays, column-major with the leading
+ * dimensions the reference uses.
+ *
+ * Two layers:
+ *   1. cpwgpu_hk_psi_c16  -- the drop-in.  Same argument list, meaning and
+ *      by-reference passing as the Fortran subroutine it replaces, host
+ *      arrays in and out.  The Fortran shim hk_psi_c16 (fortran/hk_psi_c16.f90)
+ *      forwards to it through iso_c_binding.
+ *   2. cpwgpu_plan_*      -- the device-resident path: set up a k-point once
+ *      (index maps, potential, projectors), then apply H to device arrays
+ *      on a caller stream.  The drop-in is built on it.
+ *
+ * Every function returns CPWGPU_OK (0) or one of the CPWGPU_ERR_* codes;
+ * cpwgpu_last_error() gives a message for the calling thread.
+ */
+#ifndef CPWGPU_H
+#define CPWGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference stops the program in these situations; the
+ * Fortran shim turns a nonzero code into a message on unit 6 and STOP. */
+#define CPWGPU_OK 0
+#define CPWGPU_ERR_NG 1          /* ng > mxdgve                  Src/hk_psi/reference/hk_psi_c16.f90:108     */
+#define CPWGPU_ERR_MXDSCR 2      /* FFT work size > mxdscr       Src/hk_psi/reference/hk_psi_c16.f90:112-119 */
+#define CPWGPU_ERR_WRAP 3        /* sphere wraps the FFT mesh    Src/hk_psi/reference/hk_psi_c16.f90:162-172 */
+#define CPWGPU_ERR_MTXD 4        /* mtxd > mxddim or mtxd < 0                                                 */
+#define CPWGPU_ERR_MODE 5        /* bad precision / option value                                              */
+#define CPWGPU_ERR_SIZE 6        /* FFT mesh size not supported  Src/fft/_cfft/cfft_wf_c16.f:95-102         */
+#define CPWGPU_ERR_ARG 7         /* other invalid argument (null pointer, nanl > mxdanl, ...)                 */
+#define CPWGPU_ERR_HIP 10        /* HIP runtime failure                                                       */
+#define CPWGPU_ERR_NOMEM 11      /* device allocation failed                                                  */
+
+/* Arithmetic of the FFT / pointwise / projector stages. */
+#define CPWGPU_PREC_FP64 0       /* complex128 throughout                                                     */
+#define CPWGPU_PREC_FP32 1       /* complex64 FFTs, V(r) product and projector GEMMs with FP32 accumulation;
+                                    kinetic term, psi and H|psi> stay FP64                                  */
+
+typedef struct cpwgpu_plan_s *cpwgpu_plan_t;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Select the device used by later calls from this process (default 0). */
+int cpwgpu_init(int device);
+/* Free every cached plan and device buffer. */
+int cpwgpu_finalize(void);
+/* Message describing the last error of the calling thread ("" if none). */
+const char *cpwgpu_last_error(void);
+/* Precision of cpwgpu_hk_psi_c16 (CPWGPU_PREC_*); default FP64. */
+int cpwgpu_set_precision(int prec);
+
+/* ---- drop-in ------------------------------------------------------------ */
+
+/* Replaces Src/hk_psi/reference/hk_psi_c16.f90:28-241 (subroutine hk_psi_c16).
+ * Arguments are the Fortran dummies, by reference:
+ *   mtxd, neig          basis size and number of bands
+ *   psi(mxddim,neig)    complex(8) in          hpsi(mxddim,neig) complex(8) out
+ *                       (rows 1..mtxd written, rows > mtxd untouched)
+ *   lnewanl             logical, read only (never written: callers pass literals)
+ *   ng, kgv(3,mxdgve)   G list;  isort(mxddim) 1-based row -> G index
+ *   ekpg(mxddim)        1/2|k+G|^2;  vscr(mxdscr) V(r), index (k3*n2+k2)*id+k1
+ *   kmscr(7)            (4:6) = n1,n2,n3 mesh, (7) = id leading dimension
+ *   anlga(mxddim,mxdanl), xnlkb(mxdanl), nanl    KB projectors
+ * neig = 0 is a no-op; nanl = 0 skips the non-local term. */
+int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, double *hpsi,
+                      const int *lnewanl, const int *ng, const int *kgv, const double *ekpg,
+                      const int *isort, const double *vscr, const int *kmscr, const double *anlga,
+                      const double *xnlkb, const int *nanl, const int *mxddim, const int *mxdbnd,
+                      const int *mxdanl, const int *mxdgve, const int *mxdscr);
+
+/* ---- device-resident plan ------------------------------------------------ */
+
+/* Builds the index maps of one k-point from host arrays (the hk_psi_c16
+ * inputs kgv, isort, ekpg, kmscr; same meaning and checks as the drop-in,
+ * Src/hk_psi/reference/hk_psi_c16.f90:105-172). */
+int cpwgpu_plan_create(cpwgpu_plan_t *plan, int mtxd, int ng, const int *kgv, const double *ekpg,
+                       const int *isort, const int *kmscr, int mxddim, int mxdgve);
+int cpwgpu_plan_destroy(cpwgpu_plan_t plan);
+/* V(r) from host memory (mxdscr doubles, layout of vscr). */
+int cpwgpu_plan_set_potential(cpwgpu_plan_t plan, const double *vscr, int mxdscr);
+/* Projectors from host memory: anlga(mxddim, nanl) complex(8), xnlkb(nanl). */
+int cpwgpu_plan_set_projectors(cpwgpu_plan_t plan, const double *anlga, const double *xnlkb,
+                               int nanl, int mxddim);
+/* Same from device memory (anlga complex128 with leading dimension lda). */
+int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_dev, long lda,
+                                      const double *xnlkb_dev, int nanl);
+/* H|psi> on device arrays: psi_dev(ldp, neig), hpsi_dev(ldh, neig) complex128,
+ * rows 0..mtxd-1 of hpsi written.  stream is a hipStream_t (NULL = the null
+ * stream).  Returns when the work is enqueued. */
+int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                      double *hpsi_dev, long ldh, int prec, void *stream);
+/* Sizes of the plan: out[0..9] = mtxd, n1, n2, n3, id, kd1, kd2, kd3, ncol, nanl. */
+int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
+
+/* ---- measurement hook (no counterpart in the reference) ------------------- */
+
+/* Kernels of one H|psi> application, in launch order. */
+#define CPWGPU_K_COL_BWD 0       /* scatter + inverse FFT along z                  */
+#define CPWGPU_K_PLANE 1         /* y/x FFTs with the V(r) product                 */
+#define CPWGPU_K_COL_FWD 2       /* forward FFT along z + gather + kinetic         */
+#define CPWGPU_K_PROJ 3          /* <beta|psi> split-K GEMM                        */
+#define CPWGPU_K_PROJ_REDUCE 4   /* split-K sum and D scaling                      */
+#define CPWGPU_K_BACK 5          /* hpsi += beta D<beta|psi> GEMM                  */
+#define CPWGPU_NKERNEL 6
+/* When on, later applies record a HIP event pair around each launch of the
+ * kernels selected by mask (bit k = kernel k) on the launch stream. */
+int cpwgpu_profile(int on, unsigned mask);
+/* Waits for the recorded events; ms[k], launches[k] (k < n) = summed
+ * durations and launch counts per kernel since the previous read. */
+int cpwgpu_profile_read(double *ms, long *launches, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

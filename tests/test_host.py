@@ -1,0 +1,107 @@
+"""Host-side checks that need no GPU: the synthetic workloads have the sizes of
+the benchmark configurations, the C-ABI library exports every declared entry
+point, and argument errors are reported with the reference's conditions."""
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+from cpw2000_amd import synth
+
+
+@pytest.mark.parametrize("cfg,mtxd,mesh,ncol,nanl", [
+    (0, 411, 24, 73, 8),
+    (1, 23847, 72, 997, 256),
+    (4, 25677, 80, 1049, 256),
+])
+def test_workload_sizes(cfg, mtxd, mesh, ncol, nanl):
+    d = synth.make_config(cfg, nbands=2)
+    assert d["mtxd"] == mtxd
+    assert tuple(d["kmscr"][3:6]) == (mesh,) * 3
+    g = d["kgv"][:, d["isort"][:mtxd] - 1]
+    n1, n2 = d["kmscr"][3:5]
+    assert len(set(zip(g[0] % n1, g[1] % n2))) == ncol
+    assert d["nanl"] == nanl
+
+
+def test_kpoint_workload_is_ragged():
+    spec = synth.config_spec(3)
+    sizes = {synth.build_kspace(spec, k)["mtxd"] for k in spec.kpts}
+    assert min(sizes) >= 23960 and max(sizes) <= 23990 and len(sizes) > 1
+
+
+def test_synthetic_inputs_are_physical():
+    d = synth.make_config(0)
+    p = d["psi"][: d["mtxd"]]
+    np.testing.assert_allclose(p.conj().T @ p, np.eye(p.shape[1]), atol=1e-13)
+    n1, n2, n3 = d["kmscr"][3:6]
+    v = d["vscr"].reshape(n3, n2, n1)
+    vg = np.fft.fftn(v) / v.size
+    assert abs(vg[0, 0, 0]) < 1e-14                      # V(0) = 0
+    assert np.abs(v).max() > 0.1
+
+
+def test_size_fft_rule():
+    # Src/ALFA/size_fft.f90: 2*k+1 -> smallest listed size
+    assert synth.size_fft([10, 17, 35])[0] == [24, 36, 72]
+    assert synth.size_fft([36, 53, 5])[0] == [80, 108, 12]
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "cpwgpu.h")).read()
+    return sorted(set(re.findall(r"\b(cpwgpu_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from cpw2000_amd import hk_psi
+
+    L = hk_psi.lib()
+    names = _declared()
+    assert set(names) == set(hk_psi.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", hk_psi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for n in names:
+        assert re.search(rf"\bT {n}\b", out), n
+
+
+def test_fortran_shim_exports_hk_psi_c16():
+    from cpw2000_amd import hk_psi
+
+    assert os.path.exists(hk_psi.FLIB_PATH)
+    out = subprocess.run(["nm", "-D", "--defined-only", hk_psi.FLIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    assert re.search(r"\bT hk_psi_c16_\b", out)
+
+
+def test_argument_errors_before_any_device_work():
+    """The drop-in reports the reference's stop conditions as status codes."""
+    from cpw2000_amd import hk_psi
+
+    d = synth.make_config(0, nbands=2)
+    with pytest.raises(hk_psi.CpwGpuError) as e:
+        hk_psi.hk_psi_c16_dict(dict(d, mxdgve=d["ng"] - 1))
+    assert e.value.code == hk_psi.ERR_NG
+    with pytest.raises(hk_psi.CpwGpuError) as e:
+        hk_psi.hk_psi_c16_dict(dict(d, mxdscr=100))
+    assert e.value.code == hk_psi.ERR_MXDSCR
+    km = d["kmscr"].copy()
+    km[3:7] = (10, 10, 10, 10)
+    with pytest.raises(hk_psi.CpwGpuError) as e:
+        hk_psi.hk_psi_c16_dict(dict(d, kmscr=km))
+    assert e.value.code == hk_psi.ERR_WRAP
+    km = d["kmscr"].copy()
+    km[3:7] = (28, 24, 24, 28)                           # 28 = 4*7: not 2^a 3^b 5^c
+    with pytest.raises(hk_psi.CpwGpuError) as e:
+        hk_psi.hk_psi_c16_dict(dict(d, kmscr=km))
+    assert e.value.code == hk_psi.ERR_SIZE
+    with pytest.raises(hk_psi.CpwGpuError) as e:
+        hk_psi.set_precision(7)
+    assert e.value.code == hk_psi.ERR_MODE
