@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/cpwgpu.h"
+#include "fft_fast.hpp"
 #include "fft_local.hpp"
 #include "nl_gemm.hpp"
 
@@ -305,6 +306,86 @@ int plane_rows(const FftGeom &g, size_t cs, size_t *bytes) {
     return rb;
 }
 
+// ---- specialised kernels: mesh lengths with a two-pass split ---------------
+#define CPW_FAST_SIZES(X) X(24) X(48) X(64) X(72) X(80) X(96) X(108) X(128)
+
+struct LocalArgs {
+    const double2 *psi;
+    long ldp;
+    double2 *hpsi;
+    long ldh;
+    int nb;
+};
+
+template <typename R, int N>
+int launch_cols(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, bool bwd, hipStream_t s) {
+    constexpr int CT = FT / (N / split_a(N));
+    const FftGeom &g = p->g;
+    const int ntiles = (g.ncol + CT - 1) / CT;
+    const dim3 grid(8 * ((a.nb + 7) / 8) * ntiles);
+    if (bwd) {
+        ProfScope ps_(CPWGPU_K_COL_BWD, s);
+        hipLaunchKernelGGL((k_col_bwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, colbuf,
+                           (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
+                           (const int *)p->ent_cz.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb);
+    } else {
+        ProfScope ps_(CPWGPU_K_COL_FWD, s);
+        hipLaunchKernelGGL((k_col_fwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, a.hpsi, a.ldh,
+                           (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
+                           (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
+                           (const double *)p->ekpg.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb,
+                           p->inv_n);
+    }
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+template <typename R, int N>
+int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t dyn = sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * (N + 1));
+    const size_t stat = sizeof(cplx<R>) * 2 * N;
+    if (dyn + stat > 160 * 1024)
+        return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS", N, N);
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_plane_fast<R, N, N>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)));
+        attr = true;
+    }
+    ProfScope ps_(CPWGPU_K_PLANE, s);
+    hipLaunchKernelGGL((k_plane_fast<R, N, N>), dim3(g.n[2] * a.nb), dim3(FT), dyn, s, colbuf,
+                       (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p,
+                       (const double2 *)p->tw[1].p, g);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// 1 if launched by the specialised kernels, 0 if the size is not served, <0 on error
+template <typename R>
+int local_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
+    const FftGeom &g = p->g;
+    bool col_ok = false, plane_ok = false;
+#define CPW_HAS(N) col_ok |= g.n[2] == N; plane_ok |= (g.n[0] == N && g.n[1] == N);
+    CPW_FAST_SIZES(CPW_HAS)
+#undef CPW_HAS
+    if (!col_ok || !plane_ok) return 0;
+    int st = CPWGPU_OK;
+#define CPW_COL(N) if (g.n[2] == N) st = launch_cols<R, N>(p, a, colbuf, true, s);
+    CPW_FAST_SIZES(CPW_COL)
+    if (st) return -st;
+#define CPW_COLF(N) if (g.n[2] == N) st = launch_cols<R, N>(p, a, colbuf, false, s);
+#define CPW_PLANE(N) if (g.n[0] == N) st = launch_plane<R, N>(p, a, colbuf, vr, s);
+    CPW_FAST_SIZES(CPW_PLANE)
+#undef CPW_PLANE
+    if (st) return -st;
+    CPW_FAST_SIZES(CPW_COLF)
+#undef CPW_COLF
+#undef CPW_COL
+    if (st) return -st;
+    return 1;
+}
+
 template <typename R>
 int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
             hipStream_t s) {
@@ -346,6 +427,10 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
         const double2 *pp = (const double2 *)psi + (long)b0 * ldp;
         double2 *hp = (double2 *)hpsi + (long)b0 * ldh;
         if (g.ncol > 0) {
+            const LocalArgs la{pp, ldp, hp, ldh, nb};
+            const int fast = local_fast<R>(p, la, colbuf, vr, s);
+            if (fast < 0) return -fast;
+            if (fast > 0) continue;
             { ProfScope ps_(CPWGPU_K_COL_BWD, s);
             hipLaunchKernelGGL((k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * nb), dim3(COL_THREADS), lcol, s,
                                pp, ldp, colbuf, (const int *)p->col_ptr.p, (const int *)p->ent_i.p,

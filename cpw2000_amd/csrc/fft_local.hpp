@@ -26,23 +26,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fft_core.hpp"
+
 namespace cpwgpu {
-
-template <typename R> struct alignas(2 * sizeof(R)) cplx {
-    R x, y;
-};
-
-template <typename R> __device__ __forceinline__ cplx<R> mk(R a, R b) { cplx<R> c; c.x = a; c.y = b; return c; }
-template <typename R> __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
-template <typename R> __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
-template <typename R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
-    return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename R> __device__ __forceinline__ cplx<R> scale(cplx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
-// i * dir * a  (dir = +-1)
-template <typename R> __device__ __forceinline__ cplx<R> rot(cplx<R> a, int dir) {
-    return dir > 0 ? mk<R>(-a.y, a.x) : mk<R>(a.y, -a.x);
-}
 
 // Geometry of one k-point's FFT problem (built on the host).
 constexpr int MAXRAD = 12;
@@ -60,51 +46,22 @@ struct FftGeom {
 // x of the compressed box index u (0..nx-1): 0..kd1, then n1-kd1..n1-1
 __device__ __forceinline__ int box_x(int u, int kd1, int nx, int n1) { return u <= kd1 ? u : n1 - nx + u; }
 
-// ---------------------------------------------------------------------------
-// Small DFTs in registers, dir = -1: exp(-i..) (forward), +1: exp(+i..)
-// ---------------------------------------------------------------------------
-template <typename R> __device__ __forceinline__ void dft2(cplx<R> *v) {
-    cplx<R> a = v[0], b = v[1];
-    v[0] = a + b;
-    v[1] = a - b;
-}
-template <typename R> __device__ __forceinline__ void dft3(cplx<R> *v, int dir) {
-    const R h = (R)0.86602540378443864676;  // sin(2 pi/3)
-    cplx<R> t1 = v[1] + v[2];
-    cplx<R> t2 = v[0] - scale(t1, (R)0.5);
-    cplx<R> t3 = rot(scale(v[1] - v[2], h), dir);
-    v[0] = v[0] + t1;
-    v[1] = t2 + t3;
-    v[2] = t2 - t3;
-}
-template <typename R> __device__ __forceinline__ void dft4(cplx<R> *v, int dir) {
-    cplx<R> a = v[0] + v[2], b = v[0] - v[2], c = v[1] + v[3], d = rot(v[1] - v[3], dir);
-    v[0] = a + c;
-    v[2] = a - c;
-    v[1] = b + d;
-    v[3] = b - d;
-}
-template <typename R> __device__ __forceinline__ void dft5(cplx<R> *v, int dir) {
-    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
-    const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
-    cplx<R> t1 = v[1] + v[4], t2 = v[2] + v[3], t3 = v[1] - v[4], t4 = v[2] - v[3];
-    cplx<R> a1 = v[0] + scale(t1, c1) + scale(t2, c2);
-    cplx<R> a2 = v[0] + scale(t1, c2) + scale(t2, c1);
-    cplx<R> b1 = rot(scale(t3, s1) + scale(t4, s2), dir);
-    cplx<R> b2 = rot(scale(t3, s2) - scale(t4, s1), dir);
-    v[0] = v[0] + t1 + t2;
-    v[1] = a1 + b1;
-    v[4] = a1 - b1;
-    v[2] = a2 + b2;
-    v[3] = a2 - b2;
-}
-
+// radix-r DFT with runtime r and direction (generic kernels)
 template <typename R> __device__ __forceinline__ void dft_r(cplx<R> *v, int r, int dir) {
-    switch (r) {
-    case 2: dft2(v); break;
-    case 3: dft3(v, dir); break;
-    case 4: dft4(v, dir); break;
-    default: dft5(v, dir); break;
+    if (dir > 0) {
+        switch (r) {
+        case 2: dft<2, 1, R>(v); break;
+        case 3: dft<3, 1, R>(v); break;
+        case 4: dft<4, 1, R>(v); break;
+        default: dft<5, 1, R>(v); break;
+        }
+    } else {
+        switch (r) {
+        case 2: dft<2, -1, R>(v); break;
+        case 3: dft<3, -1, R>(v); break;
+        case 4: dft<4, -1, R>(v); break;
+        default: dft<5, -1, R>(v); break;
+        }
     }
 }
 
