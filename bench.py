@@ -231,6 +231,17 @@ def main():
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},
     }
+    # the Fortran-facing drop-in: host arrays in and out (PCIe included)
+    hk.set_precision(hk.PREC_FP64)
+    hpsi_h = np.zeros_like(d["psi"], order="F")
+    hk.hk_psi_c16_dict(d, hpsi=hpsi_h)
+    t0 = time.perf_counter()
+    ncall = 3
+    for _ in range(ncall):
+        hk.hk_psi_c16_dict(d, hpsi=hpsi_h)
+    dt = (time.perf_counter() - t0) / ncall
+    out["dropin_host_fp64"] = {"bands_per_s": round(nb / dt, 1), "ms_per_call": round(dt * 1e3, 2),
+                               "note": "psi/H psi/V/projectors copied host<->device every call"}
     if not args.no_cpu:
         log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(d, args.cpu_seconds)

@@ -30,6 +30,7 @@ thread_local std::string g_err;
 std::mutex g_mu;
 int g_device = 0;
 int g_prec = CPWGPU_PREC_FP64;
+int g_overlap = 1;
 hipStream_t g_stream = nullptr;
 
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -309,81 +310,192 @@ int plane_rows(const FftGeom &g, size_t cs, size_t *bytes) {
 // ---- specialised kernels: mesh lengths with a two-pass split ---------------
 #define CPW_FAST_SIZES(X) X(24) X(48) X(64) X(72) X(80) X(96) X(108) X(128)
 
+enum LocalStage { ST_COL_BWD, ST_PLANE, ST_COL_FWD };
+
 struct LocalArgs {
     const double2 *psi;
     long ldp;
     double2 *hpsi;
     long ldh;
     int nb;
+    int acc;  // col_fwd adds to hpsi (non-local term already there)
 };
 
 template <typename R, int N>
-int launch_cols(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, bool bwd, hipStream_t s) {
-    constexpr int CT = FT / (N / split_a(N));
+int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, int stage,
+                hipStream_t s) {
     const FftGeom &g = p->g;
+    constexpr int CT = FT / (N / split_a(N));
     const int ntiles = (g.ncol + CT - 1) / CT;
     const dim3 grid(8 * ((a.nb + 7) / 8) * ntiles);
-    if (bwd) {
+    if (stage == ST_COL_BWD) {
         ProfScope ps_(CPWGPU_K_COL_BWD, s);
         hipLaunchKernelGGL((k_col_bwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, colbuf,
                            (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
                            (const int *)p->ent_cz.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb);
-    } else {
+    } else if (stage == ST_COL_FWD) {
         ProfScope ps_(CPWGPU_K_COL_FWD, s);
         hipLaunchKernelGGL((k_col_fwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, a.hpsi, a.ldh,
                            (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
                            (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
                            (const double *)p->ekpg.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb,
-                           p->inv_n);
+                           p->inv_n, a.acc);
+    } else {
+        const size_t dyn =
+            sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * (N + 1));
+        const size_t stat = sizeof(cplx<R>) * 2 * N;
+        if (dyn + stat > 160 * 1024)
+            return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS", N, N);
+        static bool attr = false;
+        if (!attr) {
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_fast<R, N, N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(160 * 1024 - stat)));
+            attr = true;
+        }
+        ProfScope ps_(CPWGPU_K_PLANE, s);
+        hipLaunchKernelGGL((k_plane_fast<R, N, N>), dim3(g.n[2] * a.nb), dim3(FT), dyn, s, colbuf,
+                           (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p,
+                           (const double2 *)p->tw[1].p, g);
     }
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
 }
 
-template <typename R, int N>
-int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
-    const FftGeom &g = p->g;
-    const size_t dyn = sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * (N + 1));
-    const size_t stat = sizeof(cplx<R>) * 2 * N;
-    if (dyn + stat > 160 * 1024)
-        return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS", N, N);
-    static bool attr = false;
-    if (!attr) {
-        HIPCHK(hipFuncSetAttribute((const void *)k_plane_fast<R, N, N>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)));
-        attr = true;
-    }
-    ProfScope ps_(CPWGPU_K_PLANE, s);
-    hipLaunchKernelGGL((k_plane_fast<R, N, N>), dim3(g.n[2] * a.nb), dim3(FT), dyn, s, colbuf,
-                       (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p,
-                       (const double2 *)p->tw[1].p, g);
-    HIPCHK(hipGetLastError());
-    return CPWGPU_OK;
-}
-
-// 1 if launched by the specialised kernels, 0 if the size is not served, <0 on error
-template <typename R>
-int local_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
-    const FftGeom &g = p->g;
+bool fast_sizes(const FftGeom &g) {
     bool col_ok = false, plane_ok = false;
 #define CPW_HAS(N) col_ok |= g.n[2] == N; plane_ok |= (g.n[0] == N && g.n[1] == N);
     CPW_FAST_SIZES(CPW_HAS)
 #undef CPW_HAS
-    if (!col_ok || !plane_ok) return 0;
-    int st = CPWGPU_OK;
-#define CPW_COL(N) if (g.n[2] == N) st = launch_cols<R, N>(p, a, colbuf, true, s);
-    CPW_FAST_SIZES(CPW_COL)
-    if (st) return -st;
-#define CPW_COLF(N) if (g.n[2] == N) st = launch_cols<R, N>(p, a, colbuf, false, s);
-#define CPW_PLANE(N) if (g.n[0] == N) st = launch_plane<R, N>(p, a, colbuf, vr, s);
-    CPW_FAST_SIZES(CPW_PLANE)
-#undef CPW_PLANE
-    if (st) return -st;
-    CPW_FAST_SIZES(CPW_COLF)
-#undef CPW_COLF
-#undef CPW_COL
-    if (st) return -st;
-    return 1;
+    return col_ok && plane_ok;
+}
+
+// One stage of the local term: specialised kernels when the mesh allows,
+// the generic runtime-radix kernels otherwise.
+template <typename R>
+int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, int stage,
+                hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t cs = sizeof(cplx<R>);
+    if (fast_sizes(g)) {
+        int st = CPWGPU_OK;
+        if (stage == ST_PLANE) {
+#define CPW_PL(N) if (g.n[0] == N) st = launch_fast<R, N>(p, a, colbuf, vr, stage, s);
+            CPW_FAST_SIZES(CPW_PL)
+#undef CPW_PL
+        } else {
+#define CPW_CO(N) if (g.n[2] == N) st = launch_fast<R, N>(p, a, colbuf, vr, stage, s);
+            CPW_FAST_SIZES(CPW_CO)
+#undef CPW_CO
+        }
+        return st;
+    }
+    const int ntiles = (g.ncol + COL_CT - 1) / COL_CT;
+    const size_t lcol = lds_col(g, cs);
+    size_t lplane;
+    const int rb = plane_rows(g, cs, &lplane);
+    if (lplane > 160 * 1024 || lcol > 160 * 1024)
+        return fail(CPWGPU_ERR_SIZE, "FFT mesh too large for LDS (%zu / %zu bytes)", lplane, lcol);
+    static bool attr_done = false;
+    if (!attr_done) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_col_bwd<R, COL_CT, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void *)k_col_fwd<R, COL_CT, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        HIPCHK(hipFuncSetAttribute((const void *)k_plane<R, KMAX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_done = true;
+    }
+    const double2 *tw1 = (const double2 *)p->tw[0].p, *tw2 = (const double2 *)p->tw[1].p,
+                  *tw3 = (const double2 *)p->tw[2].p;
+    if (stage == ST_COL_BWD) {
+        ProfScope ps_(CPWGPU_K_COL_BWD, s);
+        hipLaunchKernelGGL((k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
+                           a.psi, a.ldp, colbuf, (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
+                           (const int *)p->ent_cz.p, tw3, g, ntiles);
+    } else if (stage == ST_PLANE) {
+        ProfScope ps_(CPWGPU_K_PLANE, s);
+        hipLaunchKernelGGL((k_plane<R, KMAX>), dim3(g.n[2] * a.nb), dim3(PLANE_THREADS), lplane, s,
+                           colbuf, (const int *)p->col_uy.p, vr, tw1, tw2, g, rb);
+    } else {
+        ProfScope ps_(CPWGPU_K_COL_FWD, s);
+        hipLaunchKernelGGL((k_col_fwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
+                           a.psi, a.ldp, a.hpsi, a.ldh, (const cplx<R> *)colbuf,
+                           (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
+                           (const int *)p->ent_cz.p, (const double *)p->ekpg.p, tw3, g, ntiles,
+                           p->inv_n, a.acc);
+    }
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// Kleinman-Bylander term: hpsi = beta*hpsi + anlga Diag(xnlkb) anlga^H psi
+// (Src/cpw_lib/hk_psi_nl_c16.f90:78-96) on stream s.
+template <typename R>
+int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+             int beta, hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t cs = sizeof(cplx<R>);
+    const int nanl = p->nanl;
+    int st;
+    const cplx<R> *anl;
+    if (sizeof(R) == 8) {
+        anl = (const cplx<R> *)p->anl64.p;
+    } else {
+        if (!p->anl32_valid) {
+            if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda * nanl))) return st;
+            const long tot = (long)g.mtxd * nanl;
+            hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                               (const double2 *)p->anl64.p, p->lda, (cplx<float> *)p->anl32.p,
+                               p->lda, g.mtxd, nanl);
+            HIPCHK(hipGetLastError());
+            p->anl32_valid = true;
+        }
+        anl = (const cplx<R> *)p->anl32.p;
+    }
+    constexpr int BM = proj_bm<R>(), BN = proj_bn<R>();
+    const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
+    constexpr int BK = GemmCfg<R>::BK;
+    const int ksteps = (g.mtxd + BK - 1) / BK;
+    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+    nsl = (g.mtxd + kslice - 1) / kslice;
+    if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
+    if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
+    {
+        ProfScope ps_(CPWGPU_K_PROJ, s);
+        hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, p->lda,
+                           (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig, kslice);
+    }
+    const long tot = (long)nanl * neig;
+    {
+        ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
+        hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                           (const cplx<R> *)p->part.p, (const double *)p->xnl.p, (cplx<R> *)p->dhd.p,
+                           nanl, neig, nsl);
+    }
+    {
+        ProfScope ps_(CPWGPU_K_BACK, s);
+        hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+                           dim3(GEMM_THREADS), 0, s, anl, p->lda, (const cplx<R> *)p->dhd.p,
+                           (double2 *)hpsi, ldh, g.mtxd, nanl, neig, beta);
+    }
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// second stream and fork/join events: the projector GEMMs (matrix cores) run
+// beside the FFT kernels (LDS / HBM)
+hipStream_t g_aux = nullptr;
+hipEvent_t g_fork = nullptr, g_join = nullptr;
+
+int ensure_aux() {
+    if (!g_aux) {
+        HIPCHK(hipStreamCreateWithFlags(&g_aux, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&g_join, hipEventDisableTiming));
+    }
+    return CPWGPU_OK;
 }
 
 template <typename R>
@@ -391,12 +503,7 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
             hipStream_t s) {
     const FftGeom &g = p->g;
     const size_t cs = sizeof(cplx<R>);
-    const R *vr;
-    if (sizeof(R) == 8) {
-        vr = (const R *)p->v64.p;
-    } else {
-        vr = (const R *)p->v32.p;
-    }
+    const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
     if (!vr) return fail(CPWGPU_ERR_ARG, "potential not set");
     // bands per chunk so that colbuf stays below 4 GiB
     const size_t per_band = cs * (size_t)g.n[2] * g.ncolp;
@@ -404,86 +511,33 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
     int st = p->colbuf.reserve(per_band * nbc);
     if (st) return st;
     cplx<R> *colbuf = (cplx<R> *)p->colbuf.p;
-    const int ntiles = (g.ncol + COL_CT - 1) / COL_CT;
-    const size_t lcol = lds_col(g, cs);
-    size_t lplane;
-    const int rb = plane_rows(g, cs, &lplane);
-    if (lplane > 160 * 1024 || lcol > 160 * 1024)
-        return fail(CPWGPU_ERR_SIZE, "FFT mesh too large for LDS (%zu / %zu bytes)", lplane, lcol);
-    static bool attr_done[2] = {false, false};
-    if (!attr_done[sizeof(R) == 8]) {
-        HIPCHK(hipFuncSetAttribute((const void *)k_col_bwd<R, COL_CT, KMAX>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        HIPCHK(hipFuncSetAttribute((const void *)k_col_fwd<R, COL_CT, KMAX>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        HIPCHK(hipFuncSetAttribute((const void *)k_plane<R, KMAX>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_done[sizeof(R) == 8] = true;
+    const bool nl = p->nanl > 0 && g.mtxd > 0;
+    if (nl && !g_overlap) {
+        // serial: non-local term first (writes hpsi), then the local term adds
+        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, s))) return st;
+    } else if (nl) {
+        // fork: non-local term on the second stream, written into hpsi
+        if ((st = ensure_aux())) return st;
+        HIPCHK(hipEventRecord(g_fork, s));
+        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, g_aux))) return st;
+        HIPCHK(hipEventRecord(g_join, g_aux));
     }
-    const double2 *tw1 = (const double2 *)p->tw[0].p, *tw2 = (const double2 *)p->tw[1].p,
-                  *tw3 = (const double2 *)p->tw[2].p;
+    bool joined = !nl || !g_overlap;
     for (int b0 = 0; b0 < neig; b0 += nbc) {
         const int nb = std::min(nbc, neig - b0);
-        const double2 *pp = (const double2 *)psi + (long)b0 * ldp;
-        double2 *hp = (double2 *)hpsi + (long)b0 * ldh;
+        const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, nl ? 1 : 0};
         if (g.ncol > 0) {
-            const LocalArgs la{pp, ldp, hp, ldh, nb};
-            const int fast = local_fast<R>(p, la, colbuf, vr, s);
-            if (fast < 0) return -fast;
-            if (fast > 0) continue;
-            { ProfScope ps_(CPWGPU_K_COL_BWD, s);
-            hipLaunchKernelGGL((k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * nb), dim3(COL_THREADS), lcol, s,
-                               pp, ldp, colbuf, (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
-                               (const int *)p->ent_cz.p, tw3, g, ntiles); }
-            { ProfScope ps_(CPWGPU_K_PLANE, s);
-            hipLaunchKernelGGL((k_plane<R, KMAX>), dim3(g.n[2] * nb), dim3(PLANE_THREADS), lplane, s,
-                               colbuf, (const int *)p->col_uy.p, vr, tw1, tw2, g, rb); }
-            { ProfScope ps_(CPWGPU_K_COL_FWD, s);
-            hipLaunchKernelGGL((k_col_fwd<R, COL_CT, KMAX>), dim3(ntiles * nb), dim3(COL_THREADS), lcol, s,
-                               pp, ldp, hp, ldh, (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
-                               (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
-                               (const double *)p->ekpg.p, tw3, g, ntiles, p->inv_n); }
-            HIPCHK(hipGetLastError());
+            if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, s))) return st;
+            if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, s))) return st;
         }
-    }
-    // non-local term
-    const int nanl = p->nanl;
-    if (nanl > 0 && g.mtxd > 0) {
-        const cplx<R> *anl;
-        if (sizeof(R) == 8) {
-            anl = (const cplx<R> *)p->anl64.p;
-        } else {
-            if (!p->anl32_valid) {
-                if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda * nanl))) return st;
-                const long tot = (long)g.mtxd * nanl;
-                hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                                   (const double2 *)p->anl64.p, p->lda, (cplx<float> *)p->anl32.p,
-                                   p->lda, g.mtxd, nanl);
-                HIPCHK(hipGetLastError());
-                p->anl32_valid = true;
-            }
-            anl = (const cplx<R> *)p->anl32.p;
+        if (!joined) {
+            // join: the gather adds the local and kinetic terms to the non-local one
+            HIPCHK(hipStreamWaitEvent(s, g_join, 0));
+            joined = true;
         }
-        const int tm = (nanl + PJ_BM - 1) / PJ_BM, tn = (neig + PJ_BN - 1) / PJ_BN;
-        const int ksteps = (g.mtxd + PJ_BK - 1) / PJ_BK;
-        int nsl = std::max(1, std::min(ksteps, (1024 + tm * tn - 1) / (tm * tn)));
-        const int kslice = ((ksteps + nsl - 1) / nsl) * PJ_BK;
-        nsl = (g.mtxd + kslice - 1) / kslice;
-        if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
-        if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
-        { ProfScope ps_(CPWGPU_K_PROJ, s);
-        hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, p->lda,
-                           (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig, kslice); }
-        const long tot = (long)nanl * neig;
-        { ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
-        hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                           (const cplx<R> *)p->part.p, (const double *)p->xnl.p, (cplx<R> *)p->dhd.p,
-                           nanl, neig, nsl); }
-        { ProfScope ps_(CPWGPU_K_BACK, s);
-        hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
-                           dim3(GEMM_THREADS), 0, s, anl, p->lda, (const cplx<R> *)p->dhd.p,
-                           (double2 *)hpsi, ldh, g.mtxd, nanl, neig); }
-        HIPCHK(hipGetLastError());
+        if (g.ncol > 0 && (st = local_stage<R>(p, la, colbuf, vr, ST_COL_FWD, s))) return st;
     }
     return CPWGPU_OK;
 }
@@ -595,6 +649,13 @@ int cpwgpu_finalize(void) {
     g_drop.key_mtxd = -1;
     g_drop.psi.release();
     g_drop.hpsi.release();
+    if (g_aux) {
+        HIPCHK(hipStreamSynchronize(g_aux));
+        HIPCHK(hipStreamDestroy(g_aux));
+        HIPCHK(hipEventDestroy(g_fork));
+        HIPCHK(hipEventDestroy(g_join));
+        g_aux = nullptr;
+    }
     if (g_stream) {
         HIPCHK(hipStreamSynchronize(g_stream));
         HIPCHK(hipStreamDestroy(g_stream));
@@ -604,6 +665,11 @@ int cpwgpu_finalize(void) {
 }
 
 const char *cpwgpu_last_error(void) { return g_err.c_str(); }
+
+int cpwgpu_set_overlap(int on) {
+    g_overlap = on ? 1 : 0;
+    return CPWGPU_OK;
+}
 
 int cpwgpu_set_precision(int prec) {
     if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32)

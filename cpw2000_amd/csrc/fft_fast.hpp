@@ -117,7 +117,7 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     const double2 *__restrict__ psi, long ldp, double2 *__restrict__ hpsi, long ldh,
     const cplx<R> *__restrict__ colbuf, const int *__restrict__ col_ptr,
     const int *__restrict__ ent_i, const int *__restrict__ ent_cz, const double *__restrict__ ekpg,
-    const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb, double inv_n) {
+    const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb, double inv_n, int acc) {
     constexpr int A = split_a(N3), B = N3 / A;
     constexpr int CT = FT / B;
     constexpr int LS = N3 + 1;
@@ -147,6 +147,11 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
         double2 h;
         h.x = (double)v.x * inv_n + ek * p.x;
         h.y = (double)v.y * inv_n + ek * p.y;
+        if (acc) {  // the non-local term is already in hpsi
+            const double2 o = hb[i];
+            h.x += o.x;
+            h.y += o.y;
+        }
         hb[i] = h;
     }
 }

@@ -260,7 +260,7 @@ __global__ __launch_bounds__(COL_THREADS) void k_col_fwd(
     const double2 *__restrict__ psi, long ldp, double2 *__restrict__ hpsi, long ldh,
     const cplx<R> *__restrict__ colbuf, const int *__restrict__ col_ptr,
     const int *__restrict__ ent_i, const int *__restrict__ ent_cz, const double *__restrict__ ekpg,
-    const double2 *__restrict__ tw3, FftGeom g, int ntiles, double inv_n) {
+    const double2 *__restrict__ tw3, FftGeom g, int ntiles, double inv_n, int acc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n3 = g.n[2];
     const int ls = n3 + 1;
@@ -291,6 +291,11 @@ __global__ __launch_bounds__(COL_THREADS) void k_col_fwd(
         double2 h;
         h.x = (double)v.x * inv_n + ek * p.x;
         h.y = (double)v.y * inv_n + ek * p.y;
+        if (acc) {  // the non-local term is already in hpsi
+            const double2 o = hb[i];
+            h.x += o.x;
+            h.y += o.y;
+        }
         hb[i] = h;
     }
 }

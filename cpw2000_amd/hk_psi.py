@@ -40,6 +40,7 @@ PREC_FP32 = 1
 # every symbol declared in include/cpwgpu.h
 EXPORTS = (
     "cpwgpu_init", "cpwgpu_finalize", "cpwgpu_last_error", "cpwgpu_set_precision",
+    "cpwgpu_set_overlap",
     "cpwgpu_hk_psi_c16", "cpwgpu_plan_create", "cpwgpu_plan_destroy",
     "cpwgpu_plan_set_potential", "cpwgpu_plan_set_projectors",
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
@@ -106,6 +107,12 @@ def _iref(v):
 
 def set_precision(prec):
     _check(lib().cpwgpu_set_precision(int(prec)))
+
+
+def set_overlap(on):
+    """Non-local GEMMs on a second stream beside the FFT kernels (default) or
+    every kernel in sequence on the caller's stream."""
+    _check(lib().cpwgpu_set_overlap(1 if on else 0))
 
 
 def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr, anlga, xnlkb,

@@ -55,6 +55,10 @@ int cpwgpu_finalize(void);
 const char *cpwgpu_last_error(void);
 /* Precision of cpwgpu_hk_psi_c16 (CPWGPU_PREC_*); default FP64. */
 int cpwgpu_set_precision(int prec);
+/* 1 (default): the non-local GEMMs run on a second stream beside the FFT
+ * kernels (fork/join with events on the caller's stream); 0: every kernel on
+ * the caller's stream, one after the other.  Same work and result. */
+int cpwgpu_set_overlap(int on);
 
 /* ---- drop-in ------------------------------------------------------------ */
 
