@@ -21,6 +21,7 @@
 #include "fft_fast.hpp"
 #include "fft_local.hpp"
 #include "nl_gemm.hpp"
+#include "subspace.hpp"
 
 using namespace cpwgpu;
 
@@ -624,6 +625,86 @@ int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int
 
 }  // namespace
 
+namespace {
+
+// ---- FP64 subspace workspace ------------------------------------------------
+struct Subspace {
+    DevBuf s, m, w, v, part, pos, rot, info, eig, res, tmp;
+} g_ss;
+
+int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long ldh, int rotate,
+                  double *eig_h, double *res_h, hipStream_t st) {
+    int rc;
+    const size_t mat = 16 * (size_t)nb * nb;
+    if ((rc = g_ss.s.reserve(mat)) || (rc = g_ss.m.reserve(mat)) || (rc = g_ss.w.reserve(mat)) ||
+        (rc = g_ss.v.reserve(mat)))
+        return rc;
+    const int np = (nb + 1) & ~1;
+    if ((rc = g_ss.pos.reserve(4 * (size_t)np + 64)) || (rc = g_ss.rot.reserve(8 * (3 * (size_t)np + 8))) ||
+        (rc = g_ss.info.reserve(64)) || (rc = g_ss.eig.reserve(8 * (size_t)nb)) ||
+        (rc = g_ss.res.reserve(8 * (size_t)nb)))
+        return rc;
+    // Gram matrices on the matrix cores: split-K over the plane waves
+    constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
+    const int tm = (nb + BM - 1) / BM, tn = (nb + BN - 1) / BN;
+    const int ksteps = (mtxd + BK - 1) / BK;
+    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+    nsl = (mtxd + kslice - 1) / kslice;
+    if ((rc = g_ss.part.reserve(16 * (size_t)nsl * nb * nb))) return rc;
+    const long tot = (long)nb * nb;
+    double2 *S = (double2 *)g_ss.s.p, *M = (double2 *)g_ss.m.p, *W = (double2 *)g_ss.w.p,
+            *V = (double2 *)g_ss.v.p;
+    for (int which = 0; which < 2; which++) {
+        const double2 *y = which == 0 ? psi : hpsi;
+        const long ldy = which == 0 ? ldp : ldh;
+        hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
+                           (const cplx<double> *)psi, ldp, y, ldy, (cplx<double> *)g_ss.part.p, mtxd,
+                           nb, nb, kslice);
+        hipLaunchKernelGGL(k_proj_reduce<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           (const cplx<double> *)g_ss.part.p, (const double *)nullptr,
+                           (cplx<double> *)(which == 0 ? S : M), nb, nb, nsl);
+    }
+    int *dinfo = (int *)g_ss.info.p;
+    hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(SS_THREADS), 0, st, S, nb, dinfo);
+    hipLaunchKernelGGL(k_to_standard, dim3(1), dim3(SS_THREADS), 0, st, (const double2 *)S, M, W, nb);
+    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(SS_THREADS), 0, st, M, V, (double *)g_ss.eig.p, nb, 60,
+                       1e-15, (int *)g_ss.pos.p, (double *)g_ss.rot.p, dinfo + 1);
+    hipLaunchKernelGGL(k_back_substitute, dim3(1), dim3(SS_THREADS), 0, st, (const double2 *)S, V, nb);
+    HIPCHK(hipGetLastError());
+    int hinfo[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof hinfo, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (hinfo[0]) return fail(CPWGPU_ERR_NOTPD, "overlap matrix not positive definite (pivot %d)", hinfo[0]);
+    if (rotate) {
+        if ((rc = g_ss.tmp.reserve(16 * (size_t)mtxd * nb))) return rc;
+        double2 *T = (double2 *)g_ss.tmp.p;
+        const long cnt = (long)mtxd * nb;
+        for (int which = 0; which < 2; which++) {
+            double2 *x = which == 0 ? psi : hpsi;
+            const long ldx = which == 0 ? ldp : ldh;
+            hipLaunchKernelGGL(k_back<double>, dim3((mtxd + BP_BI - 1) / BP_BI, (nb + BP_BN - 1) / BP_BN),
+                               dim3(GEMM_THREADS), 0, st, (const cplx<double> *)x, ldx,
+                               (const cplx<double> *)V, T, (long)mtxd, mtxd, nb, nb, 0);
+            hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                               (const double2 *)T, (long)mtxd, x, ldx, mtxd, nb);
+        }
+    }
+    if (res_h) {
+        // residual of the (rotated) pairs; without rotation it uses psi, hpsi as given
+        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, (const double2 *)psi, ldp,
+                           (const double2 *)hpsi, ldh, (const double *)g_ss.eig.p, (double *)g_ss.res.p,
+                           mtxd);
+        HIPCHK(hipMemcpyAsync(res_h, g_ss.res.p, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(eig_h, g_ss.eig.p, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CPWGPU_OK;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -756,6 +837,18 @@ int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long 
     if (prec == CPWGPU_PREC_FP64) return apply_t<double>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s);
     if (prec == CPWGPU_PREC_FP32) return apply_t<float>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s);
     return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+}
+
+int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hpsi_dev, long ldh,
+                         int rotate, double *eig, double *res, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (mtxd < 1 || nb < 1 || !psi_dev || !hpsi_dev || !eig || ldp < mtxd || ldh < mtxd)
+        return fail(CPWGPU_ERR_ARG, "bad Rayleigh-Ritz arguments");
+    if (nb > mtxd) return fail(CPWGPU_ERR_ARG, "nb = %d > mtxd = %d", nb, mtxd);
+    int st = ensure_stream();
+    if (st) return st;
+    return rayleigh_ritz(mtxd, nb, (double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, rotate, eig, res,
+                         (hipStream_t)stream);
 }
 
 int cpwgpu_profile(int on, unsigned mask) {

@@ -210,7 +210,7 @@ __global__ void k_proj_reduce(const cplx<R> *__restrict__ part, const double *__
         sr += v.x;
         si += v.y;
     }
-    const R x = (R)xnl[a];
+    const R x = xnl ? (R)xnl[a] : (R)1;  // no scaling for plain Gram matrices
     dhd[t] = mk<R>(x * sr, x * si);
 }
 

@@ -31,6 +31,7 @@ ERR_MTXD = 4
 ERR_MODE = 5
 ERR_SIZE = 6
 ERR_ARG = 7
+ERR_NOTPD = 8
 ERR_HIP = 10
 ERR_NOMEM = 11
 
@@ -44,7 +45,7 @@ EXPORTS = (
     "cpwgpu_hk_psi_c16", "cpwgpu_plan_create", "cpwgpu_plan_destroy",
     "cpwgpu_plan_set_potential", "cpwgpu_plan_set_projectors",
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
-    "cpwgpu_profile", "cpwgpu_profile_read",
+    "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz",
 )
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
@@ -87,6 +88,8 @@ def lib():
         L.cpwgpu_plan_apply.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_plan_info.argtypes = [vp, ip]
         L.cpwgpu_profile.argtypes = [C.c_int, C.c_uint]
+        L.cpwgpu_rayleigh_ritz.argtypes = [C.c_int, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int,
+                                           dp, dp, vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
         _lib = L
     return _lib
@@ -170,6 +173,34 @@ def profile_read():
     return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS)}
 
 
+def _ld(t):
+    """Leading dimension of a (bands, ld) tensor whose rows are contiguous."""
+    if t.dim() != 2 or (t.shape[1] > 1 and t.stride(1) != 1):
+        raise ValueError("expected a 2-D tensor with contiguous rows (one band per row)")
+    return t.shape[1] if t.shape[0] == 1 else t.stride(0)
+
+
+def rayleigh_ritz(psi_t, hpsi_t, rotate=True, residuals=True, stream=None):
+    """FP64 Rayleigh-Ritz of the bands in psi_t with hpsi_t = H psi_t (complex128
+    device tensors shaped (nb, ld), rows 0..mtxd-1 used with mtxd = the size
+    of dimension 1).  Cholesky orthonormalisation, Jacobi eigensolve; with
+    rotate the tensors become the Ritz vectors and H applied to them.
+    Returns (eig, res) numpy arrays (res = ||H psi_n - eig_n psi_n||^2)."""
+    import torch
+
+    nb, mtxd = psi_t.shape
+    eig = np.zeros(nb)
+    res = np.zeros(nb) if residuals else None
+    if stream is None:
+        stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+    _check(lib().cpwgpu_rayleigh_ritz(
+        int(mtxd), int(nb), C.c_void_p(psi_t.data_ptr()), _ld(psi_t),
+        C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t), 1 if rotate else 0,
+        eig.ctypes.data_as(C.c_void_p), res.ctypes.data_as(C.c_void_p) if residuals else None,
+        C.c_void_p(stream)))
+    return eig, res
+
+
 class Plan:
     """One k-point resident on the device (see cpwgpu_plan_* in include/cpwgpu.h)."""
 
@@ -226,8 +257,8 @@ class Plan:
         if stream is None:
             stream = torch.cuda.current_stream(psi_t.device).cuda_stream
         _check(lib().cpwgpu_plan_apply(self._h, int(neig), C.c_void_p(psi_t.data_ptr()),
-                                       int(psi_t.stride(0)), C.c_void_p(hpsi_t.data_ptr()),
-                                       int(hpsi_t.stride(0)), int(prec), C.c_void_p(stream)))
+                                       _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
+                                       int(prec), C.c_void_p(stream)))
         return hpsi_t
 
     def close(self):

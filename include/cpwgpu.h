@@ -35,6 +35,7 @@ extern "C" {
 #define CPWGPU_ERR_MODE 5        /* bad precision / option value                                              */
 #define CPWGPU_ERR_SIZE 6        /* FFT mesh size not supported  Src/fft/_cfft/cfft_wf_c16.f:95-102         */
 #define CPWGPU_ERR_ARG 7         /* other invalid argument (null pointer, nanl > mxdanl, ...)                 */
+#define CPWGPU_ERR_NOTPD 8       /* overlap matrix of the subspace not positive definite                      */
 #define CPWGPU_ERR_HIP 10        /* HIP runtime failure                                                       */
 #define CPWGPU_ERR_NOMEM 11      /* device allocation failed                                                  */
 
@@ -102,6 +103,20 @@ int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long 
                       double *hpsi_dev, long ldh, int prec, void *stream);
 /* Sizes of the plan: out[0..9] = mtxd, n1, n2, n3, id, kd1, kd2, kd3, ncol, nanl. */
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
+
+/* ---- FP64 subspace step ----------------------------------------------------- */
+
+/* Rayleigh-Ritz of nb vectors in FP64 on device arrays psi_dev(ldp, nb) and
+ * hpsi_dev(ldh, nb) = H psi (complex128): S = psi^H psi and M = psi^H hpsi on
+ * the matrix cores, S = L L^H (Cholesky), L^-1 M L^-H = U diag(eig) U^H
+ * (Jacobi), C = L^-H U.  If rotate != 0, psi <- psi C and hpsi <- hpsi C (the
+ * Ritz vectors, orthonormal, and H applied to them).  eig (nb, ascending) and
+ * res (nb, may be NULL: res_n = ||hpsi_n - eig_n psi_n||^2 after the
+ * rotation) are host arrays.  Returns when done.  Replaces the FP64 steps of
+ * Src/ditsp/ditsp_c16_ritz.f90:157-287 (orthonormalisation, H_sub, zheevd,
+ * rotation) and the residual of Src/ditsp/ditsp_error.f90:70-76. */
+int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hpsi_dev, long ldh,
+                         int rotate, double *eig, double *res, void *stream);
 
 /* ---- measurement hook (no counterpart in the reference) ------------------- */
 
