@@ -153,9 +153,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
-              "H|psi> of independent bands/k-points shards with no communication "
-              "(see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): H|psi> of independent "
+              "bands and k-points shards across GPUs with no communication inside "
+              "the apply, so the one-GPU rate is the per-GPU figure.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -22,6 +22,7 @@
 #include "fft_local.hpp"
 #include "nl_gemm.hpp"
 #include "subspace.hpp"
+#include "forces.hpp"
 
 using namespace cpwgpu;
 
@@ -176,11 +177,12 @@ struct cpwgpu_plan_s {
     bool anl32_valid = false;
     // workspaces
     DevBuf colbuf, part, dhd;
+    DevBuf gw;                // g_j of each plane wave as doubles [3][mtxd] (force projections)
     ~cpwgpu_plan_s() {
         col_ptr.release(); col_uy.release(); ent_i.release(); ent_cz.release(); ekpg.release();
         for (auto &t : tw) t.release();
         v64.release(); v32.release(); anl64.release(); anl32.release(); xnl.release();
-        colbuf.release(); part.release(); dhd.release();
+        colbuf.release(); part.release(); dhd.release(); gw.release();
     }
 };
 
@@ -274,10 +276,15 @@ int build_plan(cpwgpu_plan_s *p, int mtxd, int ng, const int *kgv, const double 
         ei[e] = i;
         ecz[e] = (c << 12) | kz[i];
     }
+    {
+        std::vector<double> gwh(3 * (size_t)std::max(mtxd, 1));
+        for (int i = 0; i < mtxd; i++)
+            for (int j = 0; j < 3; j++) gwh[(size_t)j * mtxd + i] = (double)kgv[3L * (isort[i] - 1) + j];
+        if ((st = ensure_stream()) || (st = upload(p->gw, gwh.data(), gwh.size()))) return st;
+    }
     p->key_pts.resize(mtxd);
     for (int i = 0; i < mtxd; i++) p->key_pts[i] = ((long)kz[i] * ns[1] + ky[i]) * id + kx[i];
 
-    if ((st = ensure_stream())) return st;
     if ((st = upload(p->col_ptr, cptr.data(), cptr.size()))) return st;
     if ((st = upload(p->col_uy, uy.data(), uy.size()))) return st;
     if ((st = upload(p->ent_i, ei.data(), ei.size()))) return st;
@@ -343,7 +350,7 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
                            p->inv_n, a.acc);
     } else {
         const size_t dyn =
-            sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * (N + 1));
+            sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * row_stride<N>());
         const size_t stat = sizeof(cplx<R>) * 2 * N;
         if (dyn + stat > 160 * 1024)
             return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS", N, N);
@@ -703,6 +710,158 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
     return CPWGPU_OK;
 }
 
+// ---- atomic forces ------------------------------------------------------------
+struct ForceWs {
+    DevBuf t2, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p;
+} g_fw;
+
+template <int N>
+int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *occ, int nb, int bpg,
+                   int ngrp, int Q1, int Q2, double2 *t2, hipStream_t s) {
+    const FftGeom &g = p->g;
+    constexpr int RB = plane_rows_fast<N>();
+    const size_t dyn = 16 * ((size_t)N * (g.nx | 1) + (size_t)RB * row_stride<N>()) + 8 * (size_t)N * N;
+    const size_t stat = 16 * 2 * (size_t)N;
+    if (dyn + stat > 160 * 1024) return fail(CPWGPU_ERR_SIZE, "density plane %d does not fit in LDS", N);
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_density_fast<N, N>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)));
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_density_fast<N, N>), dim3(g.n[2] * ngrp), dim3(FT), dyn, s, colbuf,
+                       (const int *)p->col_uy.p, occ, (const double2 *)p->tw[0].p,
+                       (const double2 *)p->tw[1].p, g, nb, bpg, Q1, Q2, t2);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const double *occ_h,
+                int natom, const double *rat, const double *avec, const int *proj_atom, int nq,
+                const int *qv, int nspec, const int *species, const double *fq, double *force,
+                double *enl, hipStream_t s) {
+    const FftGeom &g = p->g;
+    if (!fast_sizes(g)) return fail(CPWGPU_ERR_SIZE, "forces need a mesh with a two-pass split");
+    int rc;
+    std::vector<double> fcov(3 * (size_t)natom, 0.0);
+    // local part ------------------------------------------------------------
+    if (nq > 0) {
+        int Q1 = 0, Q2 = 0;
+        for (int k = 0; k < nq; k++) {
+            Q1 = std::max(Q1, std::abs(qv[3 * k]));
+            Q2 = std::max(Q2, std::abs(qv[3 * k + 1]));
+            if (2 * std::abs(qv[3 * k]) >= g.n[0] || 2 * std::abs(qv[3 * k + 1]) >= g.n[1] ||
+                2 * std::abs(qv[3 * k + 2]) >= g.n[2])
+                return fail(CPWGPU_ERR_ARG, "q vector %d outside the mesh", k);
+        }
+        if (2 * Q1 + 1 > (g.nx | 1) || 2 * Q2 + 1 > g.n[1])
+            return fail(CPWGPU_ERR_SIZE, "local potential q box wider than the wave-function box");
+        const int bpg = 8, ngrp = (neig + bpg - 1) / bpg;
+        const size_t per_band = 16 * (size_t)g.n[2] * g.ncolp;
+        if ((rc = p->colbuf.reserve(per_band * neig))) return rc;
+        const size_t t2n = (size_t)ngrp * g.n[2] * (2 * Q2 + 1) * (2 * Q1 + 1);
+        if ((rc = g_fw.t2.reserve(16 * t2n)) || (rc = g_fw.rho.reserve(16 * (size_t)nq)) ||
+            (rc = upload(g_fw.qv, qv, 3 * (size_t)nq)) ||
+            (rc = upload(g_fw.fq, fq, (size_t)nq * nspec)) || (rc = upload(g_fw.rat, rat, 3 * (size_t)natom)) ||
+            (rc = upload(g_fw.species, species, (size_t)natom)) || (rc = g_fw.fl.reserve(24 * (size_t)natom)))
+            return rc;
+        const double *occ = nullptr;
+        if (occ_h) {
+            if ((rc = upload(g_fw.occ, occ_h, (size_t)neig))) return rc;
+            occ = (const double *)g_fw.occ.p;
+        }
+        HIPCHK(hipStreamSynchronize(g_stream));  // uploads went through the library stream
+        const LocalArgs la{psi, ldp, nullptr, 0, neig, 0};
+        if ((rc = local_stage<double>(p, la, (cplx<double> *)p->colbuf.p, (const double *)p->v64.p,
+                                      ST_COL_BWD, s)))
+            return rc;
+        rc = CPWGPU_ERR_SIZE;
+#define CPW_DEN(N) if (g.n[0] == N) rc = launch_density<N>(p, (const cplx<double> *)p->colbuf.p, occ, neig, bpg, ngrp, Q1, Q2, (double2 *)g_fw.t2.p, s);
+        CPW_FAST_SIZES(CPW_DEN)
+#undef CPW_DEN
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_rho_q, dim3((nq + 255) / 256), dim3(256), 0, s, (const double2 *)g_fw.t2.p,
+                           (const int *)g_fw.qv.p, nq, (const double2 *)p->tw[2].p, g.n[2], ngrp, Q1, Q2,
+                           p->inv_n, (double2 *)g_fw.rho.p);
+        hipLaunchKernelGGL(k_force_local, dim3(natom), dim3(256), 0, s, (const double2 *)g_fw.rho.p,
+                           (const int *)g_fw.qv.p, (const double *)g_fw.fq.p, nq,
+                           (const double *)g_fw.rat.p, (const int *)g_fw.species.p, (double *)g_fw.fl.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> fl(3 * (size_t)natom);
+        HIPCHK(hipMemcpyAsync(fl.data(), g_fw.fl.p, 24 * (size_t)natom, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < fl.size(); k++) fcov[k] += fl[k];
+    }
+    // non-local part --------------------------------------------------------
+    double e_nl = 0.0;
+    const int nanl = p->nanl;
+    if (nanl > 0) {
+        if (!proj_atom) return fail(CPWGPU_ERR_ARG, "proj_atom missing");
+        for (int a = 0; a < nanl; a++)
+            if (proj_atom[a] < 0 || proj_atom[a] >= natom) return fail(CPWGPU_ERR_ARG, "proj_atom[%d] = %d", a, proj_atom[a]);
+        const long mat = (long)nanl * neig;
+        if ((rc = g_fw.p.reserve(16 * 4 * (size_t)mat)) || (rc = upload(g_fw.patom, proj_atom, (size_t)nanl)) ||
+            (rc = g_fw.fnl.reserve(24 * (size_t)natom)) || (rc = g_fw.enl.reserve(8 * (size_t)natom)))
+            return rc;
+        const double *occ = nullptr;
+        if (occ_h) {
+            if ((rc = upload(g_fw.occ, occ_h, (size_t)neig))) return rc;
+            occ = (const double *)g_fw.occ.p;
+        }
+        HIPCHK(hipStreamSynchronize(g_stream));
+        constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
+        const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
+        const int ksteps = (g.mtxd + BK - 1) / BK;
+        int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+        const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+        nsl = (g.mtxd + kslice - 1) / kslice;
+        if ((rc = p->part.reserve(16 * (size_t)nsl * mat))) return rc;
+        for (int j = 0; j < 4; j++) {
+            const double *w = j == 0 ? nullptr : (const double *)p->gw.p + (size_t)(j - 1) * g.mtxd;
+            hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s,
+                               (const cplx<double> *)p->anl64.p, p->lda, psi, ldp,
+                               (cplx<double> *)p->part.p, g.mtxd, nanl, neig, kslice, w);
+            hipLaunchKernelGGL(k_proj_reduce<double>, dim3((unsigned)((mat + 255) / 256)), dim3(256), 0, s,
+                               (const cplx<double> *)p->part.p, (const double *)nullptr,
+                               (cplx<double> *)g_fw.p.p + (size_t)j * mat, nanl, neig, nsl);
+        }
+        hipLaunchKernelGGL(k_force_nl, dim3(natom), dim3(256), 0, s, (const double2 *)g_fw.p.p, nanl,
+                           neig, (const double *)p->xnl.p, occ, (const int *)g_fw.patom.p,
+                           (double *)g_fw.fnl.p, (double *)g_fw.enl.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> fn(3 * (size_t)natom), en(natom);
+        HIPCHK(hipMemcpyAsync(fn.data(), g_fw.fnl.p, 24 * (size_t)natom, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(en.data(), g_fw.enl.p, 8 * (size_t)natom, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < fn.size(); k++) fcov[k] += fn[k];
+        for (int a = 0; a < natom; a++) e_nl += en[a];
+    }
+    // Cartesian: F = inv(avec)^T f_cov  (avec columns are the lattice vectors)
+    const double *A = avec;  // A[i + 3 j] = component i of a_j
+    const double det = A[0] * (A[4] * A[8] - A[7] * A[5]) - A[3] * (A[1] * A[8] - A[7] * A[2]) +
+                       A[6] * (A[1] * A[5] - A[4] * A[2]);
+    if (std::fabs(det) < 1e-300) return fail(CPWGPU_ERR_ARG, "singular lattice vectors");
+    double inv[9];  // inv = A^-1, inv[i + 3 j]
+    inv[0] = (A[4] * A[8] - A[7] * A[5]) / det;
+    inv[3] = -(A[3] * A[8] - A[6] * A[5]) / det;
+    inv[6] = (A[3] * A[7] - A[6] * A[4]) / det;
+    inv[1] = -(A[1] * A[8] - A[7] * A[2]) / det;
+    inv[4] = (A[0] * A[8] - A[6] * A[2]) / det;
+    inv[7] = -(A[0] * A[7] - A[6] * A[1]) / det;
+    inv[2] = (A[1] * A[5] - A[4] * A[2]) / det;
+    inv[5] = -(A[0] * A[5] - A[3] * A[2]) / det;
+    inv[8] = (A[0] * A[4] - A[3] * A[1]) / det;
+    for (int a = 0; a < natom; a++)
+        for (int i = 0; i < 3; i++) {
+            // (inv^T f)_i = sum_j inv[j][i] f_j = sum_j inv[j + 3 i] f_j
+            double v = 0.0;
+            for (int j = 0; j < 3; j++) v += inv[j + 3 * i] * fcov[3 * a + j];
+            force[3 * a + i] = v;
+        }
+    if (enl) *enl = e_nl;
+    return CPWGPU_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -849,6 +1008,92 @@ int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hp
     if (st) return st;
     return rayleigh_ritz(mtxd, nb, (double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, rotate, eig, res,
                          (hipStream_t)stream);
+}
+
+int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                       const double *occ, int natom, const double *rat, const double *avec,
+                       const int *proj_atom, int nq, const int *qv, int nspec, const int *species,
+                       const double *fq, double *force, double *enl) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || neig < 1 || !psi_dev || ldp < plan->mtxd || natom < 1 || !rat || !avec || !force ||
+        nq < 0 || (nq > 0 && (!qv || !species || !fq || nspec < 1)))
+        return fail(CPWGPU_ERR_ARG, "bad force arguments");
+    for (int a = 0; nq > 0 && a < natom; a++)
+        if (species[a] < 0 || species[a] >= nspec) return fail(CPWGPU_ERR_ARG, "species[%d] = %d", a, species[a]);
+    int st = ensure_stream();
+    if (st) return st;
+    return plan_forces(plan, neig, (const double2 *)psi_dev, ldp, occ, natom, rat, avec, proj_atom, nq, qv,
+                       nspec, species, fq, force, enl, g_stream);
+}
+
+}  // extern "C"
+
+struct cpwgpu_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plans, const int *neig,
+                        const double *const *psi, const long *ldp, double *const *hpsi,
+                        const long *ldh, int prec) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!graph || nk < 1 || !plans || !neig || !psi || !ldp || !hpsi || !ldh)
+        return fail(CPWGPU_ERR_ARG, "null graph argument");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    *graph = nullptr;
+    int st = ensure_stream();
+    if (st) return st;
+    for (int k = 0; k < nk; k++)
+        if (!plans[k] || neig[k] < 0 || ldp[k] < plans[k]->mtxd || ldh[k] < plans[k]->mtxd)
+            return fail(CPWGPU_ERR_ARG, "bad arguments for k-point %d", k);
+    auto apply = [&](int k, hipStream_t s) -> int {
+        if (neig[k] == 0) return CPWGPU_OK;
+        return prec == CPWGPU_PREC_FP64
+                   ? apply_t<double>(plans[k], neig[k], psi[k], ldp[k], hpsi[k], ldh[k], s)
+                   : apply_t<float>(plans[k], neig[k], psi[k], ldp[k], hpsi[k], ldh[k], s);
+    };
+    // eager pass: every buffer, attribute and FP32 projector copy exists before capture
+    const bool prof = g_prof.on;
+    g_prof.on = false;
+    for (int k = 0; k < nk; k++)
+        if ((st = apply(k, g_stream))) { g_prof.on = prof; return st; }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    hipStream_t cs = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    cpwgpu_graph_s *gr = new cpwgpu_graph_s();
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    for (int k = 0; e == hipSuccess && k < nk && st == CPWGPU_OK; k++) st = apply(k, cs);
+    hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &gr->graph) : e;
+    g_prof.on = prof;
+    if (e == hipSuccess && e2 == hipSuccess && st == CPWGPU_OK)
+        e2 = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (st != CPWGPU_OK || e != hipSuccess || e2 != hipSuccess) {
+        if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        delete gr;
+        if (st) return st;
+        return fail(CPWGPU_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    *graph = gr;
+    return CPWGPU_OK;
+}
+
+int cpwgpu_graph_launch(cpwgpu_graph_t graph, void *stream) {
+    if (!graph || !graph->exec) return fail(CPWGPU_ERR_ARG, "null graph");
+    HIPCHK(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_graph_destroy(cpwgpu_graph_t graph) {
+    if (!graph) return CPWGPU_OK;
+    hipDeviceSynchronize();
+    if (graph->exec) HIPCHK(hipGraphExecDestroy(graph->exec));
+    if (graph->graph) HIPCHK(hipGraphDestroy(graph->graph));
+    delete graph;
+    return CPWGPU_OK;
 }
 
 int cpwgpu_profile(int on, unsigned mask) {

@@ -24,17 +24,20 @@ __device__ __forceinline__ bool xcd_band_tile(int b, int ntiles, int nb, int &ba
 
 // Two-pass transform of nl lines in LDS: line l at buf + l*ldist, element j at
 // + j*estr.  Lines go in rounds of FT/B whole lines (all B work items of a
-// line in the same round); work item t < B of round line r is thread r*B + t.
+// line in the same round).  Threads are numbered line fastest (work item t of
+// round line r is thread t*LPR + r): 16 neighbouring lanes touch the same
+// position of 16 neighbouring lines, which the callers lay out at a stride
+// that is odd in 16-byte units, so every LDS access is free of bank conflicts.
 // The result is stored back in natural order.
 template <typename R, int N, int DIR>
 __device__ __forceinline__ void line_fft(cplx<R> *buf, int nl, int ldist, int estr,
                                          const cplx<R> *tw) {
     constexpr int A = split_a(N), B = N / A;
     constexpr int LPR = FT / B;
-    const int r = (int)threadIdx.x / B, t = (int)threadIdx.x - r * B;
+    const int r = (int)threadIdx.x % LPR, t = (int)threadIdx.x / LPR;
     for (int l0 = 0; l0 < nl; l0 += LPR) {
         const int l = l0 + r;
-        const bool act = r < LPR && l < nl;
+        const bool act = t < B && l < nl;
         cplx<R> *line = buf + l * ldist;
         cplx<R> v[B > A ? B : A];
         if (act && t < A) {
@@ -161,6 +164,12 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
 // V(r), forward x FFT, forward y FFT.  LDS: tw1[N1] tw2[N2] sub[N2][nxp]
 // rows[RB][N1+1] (sub/rows in dynamic shared memory, nxp depends on kd1).
 // ---------------------------------------------------------------------------
+// Row scratch of the x transforms: element p of a row at p + p/A1 (one pad per
+// A1 elements), so that the stride-A1 and stride-B1 accesses of the two passes
+// are odd strides in 16-byte units; rows RS apart.
+template <int N1> constexpr int row_stride() { return N1 + N1 / split_a(N1) + 1; }
+template <int N1> __device__ __forceinline__ int rpos(int p) { return p + p / split_a(N1); }
+
 template <typename R, int N1, int N2>
 __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
                                                    const int *__restrict__ col_uy,
@@ -169,7 +178,7 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
                                                    const double2 *__restrict__ tw2g, FftGeom g) {
     constexpr int A1 = split_a(N1), B1 = N1 / A1;
     constexpr int RB = FT / B1;
-    constexpr int RS = N1 + 1;
+    constexpr int RS = row_stride<N1>();
     __shared__ cplx<R> tw1[N1];
     __shared__ cplx<R> tw2[N2];
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -212,14 +221,14 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
             }
             dft<B1, +1, R>(v);
 #pragma unroll
-            for (int k = 0; k < B1; k++) row[t + A1 * k] = k == 0 ? v[0] : cmul(v[k], tw1[t * k]);
+            for (int k = 0; k < B1; k++) row[rpos<N1>(t + A1 * k)] = k == 0 ? v[0] : cmul(v[k], tw1[t * k]);
         }
         __syncthreads();
         if (act) {
             // pass 2 -> psi(r) at x = t + B1 q, times V(r), then the forward
             // transform's first pass on the same registers (split B1 x A1)
 #pragma unroll
-            for (int s = 0; s < A1; s++) v[s] = row[s + A1 * t];
+            for (int s = 0; s < A1; s++) v[s] = row[rpos<N1>(s + A1 * t)];
             dft<A1, +1, R>(v);
             const R *vy = vz + (long)y * g.id;
 #pragma unroll
@@ -235,12 +244,12 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
         __syncthreads();
         if (act) {
 #pragma unroll
-            for (int m = 0; m < A1; m++) row[t + B1 * m] = v[m];
+            for (int m = 0; m < A1; m++) row[rpos<N1>(t + B1 * m)] = v[m];
         }
         __syncthreads();
         if (act && t < A1) {
 #pragma unroll
-            for (int k = 0; k < B1; k++) v[k] = row[k + B1 * t];
+            for (int k = 0; k < B1; k++) v[k] = row[rpos<N1>(k + B1 * t)];
             dft<B1, -1, R>(v);
 #pragma unroll
             for (int p = 0; p < B1; p++) {

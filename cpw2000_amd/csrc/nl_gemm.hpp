@@ -67,8 +67,9 @@ template <typename R> constexpr int proj_bm() { return 2 * GemmCfg<R>::WM * 16; 
 template <typename R> constexpr int proj_bn() { return 2 * GemmCfg<R>::WN * 16; }
 
 // ---------------------------------------------------------------------------
-// k_proj: part[s][n][a] = sum_{i in slice s} conj(anlga[i,a]) psi[i,n]
-// grid (ceil(nanl/BM), ceil(neig/BN), nslice); 4 waves, each WM*16 a x WN*16 n
+// k_proj: part[s][n][a] = sum_{i in slice s} w_i conj(anlga[i,a]) psi[i,n]
+// (w_i = 1 without wrow) grid (ceil(nanl/BM), ceil(neig/BN), nslice); 4 waves,
+// each WM*16 a x WN*16 n
 // (FP32: block 128 x 128, so a K slice of anlga and psi is read at most
 // twice).  Double-buffered LDS: one barrier per K stage, the next stage's
 // global loads in flight during the current stage's MFMAs.
@@ -77,7 +78,8 @@ template <typename R>
 __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict__ anl, long lda,
                                                        const double2 *__restrict__ psi, long ldp,
                                                        cplx<R> *__restrict__ part, int mtxd,
-                                                       int nanl, int neig, int kslice) {
+                                                       int nanl, int neig, int kslice,
+                                                       const double *__restrict__ wrow = nullptr) {
     typedef typename MF<R>::v4 v4;
     constexpr int WM = GemmCfg<R>::WM, WN = GemmCfg<R>::WN;
     constexpr int BK = GemmCfg<R>::BK, KP = GemmCfg<R>::KP;
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
             const cplx<R> v = anl[abase + min(i, mtxd - 1)];
             const bool ok = ga < nanl && i < ke;
             ra[q] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+            if (wrow) ra[q] = scale(ra[q], (R)wrow[min(i, mtxd - 1)]);  // per-plane-wave weight
         }
 #pragma unroll
         for (int q = 0; q < LPB; q++) {

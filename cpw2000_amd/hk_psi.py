@@ -45,7 +45,8 @@ EXPORTS = (
     "cpwgpu_hk_psi_c16", "cpwgpu_plan_create", "cpwgpu_plan_destroy",
     "cpwgpu_plan_set_potential", "cpwgpu_plan_set_projectors",
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
-    "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz",
+    "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz", "cpwgpu_plan_forces",
+    "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
 )
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
@@ -90,6 +91,11 @@ def lib():
         L.cpwgpu_profile.argtypes = [C.c_int, C.c_uint]
         L.cpwgpu_rayleigh_ritz.argtypes = [C.c_int, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int,
                                            dp, dp, vp]
+        L.cpwgpu_plan_forces.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_int, dp, dp, vp, C.c_int,
+                                         vp, C.c_int, vp, dp, dp, dp]
+        L.cpwgpu_graph_create.argtypes = [C.POINTER(vp), C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.cpwgpu_graph_launch.argtypes = [vp, vp]
+        L.cpwgpu_graph_destroy.argtypes = [vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
         _lib = L
     return _lib
@@ -261,9 +267,80 @@ class Plan:
                                        int(prec), C.c_void_p(stream)))
         return hpsi_t
 
+    def forces(self, psi_t, rat, avec, proj_atom, floc_q=None, floc_f=None, species=None,
+               occ=None):
+        """Cartesian forces (natom, 3) Ha/bohr and E_NL for E = sum_n occ_n
+        <psi_n|V_loc + V_NL|psi_n> at fixed psi (psi_t: complex128 (nb, ld) on
+        cuda).  rat (natom, 3) fractional, avec (3, 3) lattice vectors as
+        columns, proj_atom (nanl,) atom of each projector, floc_q (nq, 3) and
+        floc_f (nq, nspec) the position-dependent local potential
+        V(q) = sum_s f_s(q) sum_{atoms of s} exp(-2 pi i q.rat), species (natom,)."""
+        import torch
+
+        torch.cuda.synchronize()
+        nb = psi_t.shape[0]
+        rat = np.ascontiguousarray(rat, np.float64)
+        natom = rat.shape[0]
+        av = np.asfortranarray(avec, np.float64)
+        pa = np.ascontiguousarray(proj_atom, np.int32)
+        force = np.zeros((natom, 3))
+        enl = C.c_double(0.0)
+        if floc_q is not None and len(floc_q):
+            q = np.ascontiguousarray(floc_q, np.int32)
+            f = np.asfortranarray(floc_f, np.float64)
+            sp = np.ascontiguousarray(species, np.int32)
+            nq, nspec = q.shape[0], f.shape[1]
+            qp, fp, spp = _p(q), _p(f), _p(sp)
+        else:
+            nq, nspec, qp, fp, spp = 0, 0, None, None, None
+        oc = None if occ is None else np.ascontiguousarray(occ, np.float64)
+        _check(lib().cpwgpu_plan_forces(
+            self._h, int(nb), C.c_void_p(psi_t.data_ptr()), _ld(psi_t),
+            None if oc is None else _p(oc), int(natom), _p(rat), _p(av), _p(pa), int(nq), qp,
+            int(nspec), spp, fp, _p(force), C.byref(enl)))
+        return force, enl.value
+
     def close(self):
         if self._h:
             lib().cpwgpu_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """H|psi> of several k-points (each its own Plan, plane-wave count and
+    bands) recorded once into a HIP graph and replayed with one launch."""
+
+    def __init__(self, plans, psis, hpsis, prec=PREC_FP64):
+        import torch
+
+        torch.cuda.synchronize()
+        nk = len(plans)
+        self._keep = (list(plans), list(psis), list(hpsis))
+        P = (C.c_void_p * nk)(*[p._h.value for p in plans])
+        ne = (C.c_int * nk)(*[t.shape[0] for t in psis])
+        ps = (C.c_void_p * nk)(*[t.data_ptr() for t in psis])
+        lp = (C.c_long * nk)(*[_ld(t) for t in psis])
+        hs = (C.c_void_p * nk)(*[t.data_ptr() for t in hpsis])
+        lh = (C.c_long * nk)(*[_ld(t) for t in hpsis])
+        self._h = C.c_void_p()
+        _check(lib().cpwgpu_graph_create(C.byref(self._h), nk, P, ne, ps, lp, hs, lh, int(prec)))
+
+    def launch(self, stream=None):
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().cpwgpu_graph_launch(self._h, C.c_void_p(stream)))
+
+    def close(self):
+        if self._h:
+            lib().cpwgpu_graph_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -139,6 +139,15 @@ def config_spec(cfg: int) -> CellSpec:
         # Gaussian local form factors v_s(q) = A_s exp(-q^2 r_s^2/2) (Ha bohr^3)
         spec.vloc_gauss = {"Si": (-8.0, 1.2), "Ge": (-9.0, 1.3)}
         return spec
+    if cfg == "si2f":
+        # the 2-atom cell with a tau-dependent local potential and moved atoms:
+        # nonzero forces, small enough for dense checks
+        spec = config_spec(0)
+        frac = spec.types[0][1] + np.array([[0.011, -0.007, 0.004], [-0.006, 0.009, 0.013]])
+        spec.types = [("Si", frac, 1.8)]
+        spec.vloc_gauss = {"Si": (-8.0, 1.2)}
+        spec.name = "Si2 fcc primitive, displaced, local Gaussian form factor"
+        return spec
     if cfg == "ortho":
         # small orthorhombic cell: n1, n2, n3 all different (layout checks)
         avec = np.diag([9.0, 11.5, 7.2])
@@ -238,6 +247,36 @@ def vloc_from_species(spec, kgv, bvec, ns, omega):
     return g, v
 
 
+def local_form_factors(spec, kgv, bvec, omega):
+    """q list (|q| <= gv, q != 0) and f_s(q) = A_s exp(-q^2 r_s^2/2)/Omega per species,
+    so that the tau-dependent local potential is
+    V(q) = sum_s f_s(q) sum_{atoms of s} exp(-2 pi i q.rat)."""
+    g = kgv.T.astype(np.int64)
+    gc = g @ bvec.T
+    g2 = np.einsum("ij,ij->i", gc, gc)
+    gl = np.sqrt(g2)
+    sel = (gl > 0) & (gl <= spec.gv)
+    g = g[sel]
+    g2 = g2[sel]
+    f = np.zeros((g.shape[0], len(spec.types)))
+    for k, (name, _frac, _rc) in enumerate(spec.types):
+        A, rs = spec.vloc_gauss[name]
+        f[:, k] = A * np.exp(-0.5 * g2 * rs * rs) / omega
+    return g.astype(np.int32), f
+
+
+def projector_atoms(spec):
+    """Atom (global index, species-major) of each projector column, in the
+    ordering of ``projectors`` (type -> l -> m -> atom)."""
+    out, base = [], 0
+    for name, frac, rc in spec.types:
+        for l in (0, 1):
+            for _m in range(2 * l + 1):
+                out.extend(base + np.arange(frac.shape[0]))
+        base += frac.shape[0]
+    return np.array(out, np.int32)
+
+
 def potential_on_mesh(gq, vq, ns, idd=None):
     """V(r_j) = sum_q V(q) exp(+2 pi i q.j/n), stored as (k3*n2+k2)*id+k1."""
     n1, n2, n3 = ns
@@ -296,13 +335,19 @@ def wavefunctions(rng, ekpg, nb, mxddim=None):
 
 
 def make_config(cfg, nbands: int | None = None, kindex: int = 0, with_psi: bool = True,
-                mxddim_pad: int = 0, kpt=None, kmax_override=None, idshift: int = 0):
+                mxddim_pad: int = 0, kpt=None, kmax_override=None, idshift: int = 0,
+                positions=None):
     """All hk_psi_c16 inputs for workload ``cfg`` (k-point ``kindex`` for cfg 3).
 
     Variants: ``kpt`` (fractional k, overrides the config's), ``kmax_override``
     (kmscr(1:3), smaller mesh), ``idshift`` (id > n1), ``mxddim_pad`` (rows
-    beyond mtxd in every column)."""
+    beyond mtxd in every column), ``positions`` (list of fractional position
+    arrays, one per species, replacing the config's: the random draws, hence
+    psi and the tau-independent part of V, stay the same)."""
     spec = config_spec(cfg)
+    if positions is not None:
+        spec.types = [(name, np.asarray(pos, float), rc)
+                      for (name, _, rc), pos in zip(spec.types, positions)]
     nb = spec.nbands if nbands is None else nbands
     k = spec.kpts[kindex] if kpt is None else kpt
     ks = build_kspace(spec, k, kmax_override, idshift)
@@ -314,10 +359,12 @@ def make_config(cfg, nbands: int | None = None, kindex: int = 0, with_psi: bool 
     seed = 20260000 + (cfg if isinstance(cfg, int) else 99)
     rng = np.random.Generator(np.random.PCG64(seed))
     gq, vq = _random_potential(rng, spec, ks["kgv"], ks["bvec"], ns)
+    floc_q, floc_f = None, None
     if spec.vloc_gauss is not None:
         gl, vl = vloc_from_species(spec, ks["kgv"], ks["bvec"], ns, omega)
         gq = np.concatenate([gq, gl])
         vq = np.concatenate([vq, vl])
+        floc_q, floc_f = local_form_factors(spec, ks["kgv"], ks["bvec"], omega)
     vscr = potential_on_mesh(gq, vq, ns, int(kmscr[6]))
     anlga, xnlkb = projectors(spec, ks["kgv"], ks["isort"], ks["kpt"], ks["bvec"], omega, mxddim)
     psi = None
@@ -333,4 +380,8 @@ def make_config(cfg, nbands: int | None = None, kindex: int = 0, with_psi: bool 
                 ekpg=ekpg, isort=isort, vscr=vscr, kmscr=kmscr, anlga=anlga, xnlkb=xnlkb,
                 nanl=anlga.shape[1], mxddim=mxddim, mxdbnd=nb, mxdanl=anlga.shape[1],
                 mxdgve=ks["ng"], mxdscr=vscr.shape[0], vq_g=gq, vq=vq, omega=omega,
-                bvec=ks["bvec"], bdot=ks["bdot"], kpt=ks["kpt"])
+                bvec=ks["bvec"], bdot=ks["bdot"], kpt=ks["kpt"], avec=spec.avec,
+                rat=np.concatenate([t[1] for t in spec.types]),
+                species=np.concatenate([np.full(t[1].shape[0], k, np.int32)
+                                        for k, t in enumerate(spec.types)]),
+                proj_atom=projector_atoms(spec), floc_q=floc_q, floc_f=floc_f)

@@ -104,6 +104,22 @@ int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long 
 /* Sizes of the plan: out[0..9] = mtxd, n1, n2, n3, id, kd1, kd2, kd3, ncol, nanl. */
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 
+/* ---- several k-points in one graph launch -------------------------------------- */
+
+typedef struct cpwgpu_graph_s *cpwgpu_graph_t;
+/* Records H|psi> for nk k-points (plans[k], neig[k] bands psi[k](ldp[k]) ->
+ * hpsi[k](ldh[k]) on device, each k with its own plane-wave count) into one
+ * HIP graph.  Each plan is applied once eagerly first (buffers, kernel
+ * attributes), then the applies are captured; the device arrays must stay
+ * valid while the graph exists.  Replaces the k-loop of
+ * Src/cpw/cpw_scf.f90:488-544 around hk_psi_c16 for a batch of k-points. */
+int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plans, const int *neig,
+                        const double *const *psi, const long *ldp, double *const *hpsi,
+                        const long *ldh, int prec);
+/* One launch of the recorded graph on stream (NULL = null stream). */
+int cpwgpu_graph_launch(cpwgpu_graph_t graph, void *stream);
+int cpwgpu_graph_destroy(cpwgpu_graph_t graph);
+
 /* ---- FP64 subspace step ----------------------------------------------------- */
 
 /* Rayleigh-Ritz of nb vectors in FP64 on device arrays psi_dev(ldp, nb) and
@@ -117,6 +133,25 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  * rotation) and the residual of Src/ditsp/ditsp_error.f90:70-76. */
 int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hpsi_dev, long ldh,
                          int rotate, double *eig, double *res, void *stream);
+
+/* ---- atomic forces ----------------------------------------------------------- */
+
+/* Forces on the atoms from E = sum_n occ_n <psi_n| V_loc + V_NL |psi_n> at
+ * fixed psi, FP64, replacing the force parts of Src/ALFA/for_str_nl_kb.f90
+ * (:286-398) and Src/ALFA/for_str_local_force.f90 (:102-145) for this energy:
+ *   psi_dev(ldp, neig)   device complex128 bands; occ (host, neig) or NULL = 1
+ *   natom, rat(3,natom)  fractional positions; avec(3,3) lattice vectors as
+ *                        columns (bohr)
+ *   proj_atom(nanl)      atom (0-based) of each projector column of the plan
+ *   nq, qv(3,nq)         q vectors (integers) of the position-dependent local
+ *                        potential V(q) = sum_s fq(q,s) sum_{atoms of s} exp(-2 pi i q.rat)
+ *   nspec, species(natom) (0-based), fq(nq, nspec) real form factors
+ * Outputs (host): force(3, natom) Cartesian Ha/bohr, enl = E_NL (may be NULL).
+ * The plan's mesh must be one of the specialised FFT sizes. */
+int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                       const double *occ, int natom, const double *rat, const double *avec,
+                       const int *proj_atom, int nq, const int *qv, int nspec, const int *species,
+                       const double *fq, double *force, double *enl);
 
 /* ---- measurement hook (no counterpart in the reference) ------------------- */
 
