@@ -15,8 +15,11 @@ FP64 throughput is reported beside it.  Both Rayleigh-Ritz steps run in FP64 on
 the GPU (cpwgpu_rayleigh_ritz); the refined error repeats the step with one
 FP64 application of H to the FP32-mixed Ritz vectors.
 
-Per-kernel durations come from HIP events recorded around each launch on the
-launch stream inside the timed loops (library measurement hook).
+Before timing, the library's autotuner picks the plane-stage FFT kernel and the
+split-K shape of the projection GEMM for each precision on this workload
+(reported under "tuned").  Per-kernel durations come from HIP events recorded
+around each launch on the launch stream inside the timed loops (library
+measurement hook).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
 """
@@ -174,6 +177,13 @@ def main():
     h64 = torch.zeros_like(psi)
     h32 = torch.zeros_like(psi)
 
+    # autotuner: plane-stage kernel and split-K target per precision, timed on
+    # this workload (part of the warm-up; every later apply uses the choice)
+    tuned = {}
+    for prec, pid, h in (("fp64", hk.PREC_FP64, h64), ("fp32", hk.PREC_FP32, h32)):
+        tuned[prec] = plan.tune(psi, h, pid)
+    log(f"bench: tuned kernels {tuned}")
+
     # accuracy of the FP32-mixed path against FP64 (same inputs, same run)
     plan.apply(psi, h64, hk.PREC_FP64)
     plan.apply(psi, h32, hk.PREC_FP32)
@@ -240,6 +250,7 @@ def main():
                                    nb, "fp32"),
         "roofline_fp64": roofline_entry(res["fp64"]["dominant"], work["fp64"],
                                         res["fp64"]["dom_ms"], nb, "fp64"),
+        "tuned": tuned,
         "kernel_ms": kernels,
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},

@@ -20,6 +20,7 @@
 #include "../../include/cpwgpu.h"
 #include "fft_fast.hpp"
 #include "fft_local.hpp"
+#include "fft_reg.hpp"
 #include "nl_gemm.hpp"
 #include "subspace.hpp"
 #include "forces.hpp"
@@ -173,8 +174,17 @@ struct cpwgpu_plan_s {
     DevBuf col_ptr, col_uy, ent_i, ent_cz, ekpg, tw[3];
     DevBuf v64, v32;          // V(r) in FP64 and FP32
     DevBuf anl64, anl32, xnl; // projectors
-    long lda = 0;             // leading dimension of anl64/anl32
+    long lda = 0;             // leading dimension of anl64
+    long lda32 = 0;           // of anl32: lda rounded up to even (16-byte aligned rows)
     bool anl32_valid = false;
+    // plane-stage kernel per precision (index 0 FP64, 1 FP32): -1 = default for
+    // the mesh, else a variant number (set by the tuner or by the caller)
+    int plane_var[2] = {-1, -1};
+    // split-K target blocks of the projection GEMM per precision (-1 = 256)
+    int splitk[2] = {-1, -1};
+    // projection GEMM kernel per precision: 0 LDS-staged k_proj, 1 k_proj_direct
+    // (-1 = default)
+    int proj_var[2] = {-1, -1};
     // workspaces
     DevBuf colbuf, part, dhd;
     DevBuf gw;                // g_j of each plane wave as doubles [3][mtxd] (force projections)
@@ -370,12 +380,98 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
     return CPWGPU_OK;
 }
 
+// ---- plane stage variants ---------------------------------------------------
+// 0: k_plane_fast (LDS two-pass lines); 1, 2: k_plane_reg shapes
+// (x lanes GX, y lanes GY, planes per block P, threads T) per precision.
+template <typename R> struct RegShape;
+template <> struct RegShape<float> {
+    static constexpr int GX[2] = {2, 4}, GY[2] = {4, 4}, P[2] = {2, 2}, T[2] = {320, 384};
+    static constexpr int W[2] = {3, 4};
+};
+template <> struct RegShape<double> {
+    static constexpr int GX[2] = {8, 8}, GY[2] = {8, 8}, P[2] = {1, 1}, T[2] = {256, 384};
+    static constexpr int W[2] = {3, 5};
+};
+constexpr int NPLANE_VAR = 3;
+
+// a register shape is built for a mesh length when both splits divide N and a
+// lane holds at most 288 bytes of one line
+template <typename R, int N, int GX, int GY> constexpr bool reg_ok() {
+    return N % GX == 0 && N % GY == 0 && first_factor(N / GX) <= 5 && first_factor(N / GY) <= 5 &&
+           2 * sizeof(R) * (N / GX) <= 288 && 2 * sizeof(R) * (N / GY) <= 288;
+}
+
+// LDS row stride of the register variant's sub-plane for nx box columns
+inline int reg_stride(int nx) { return nx | 1; }
+
+template <typename R, int N, int V>
+int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr,
+                     hipStream_t s) {
+    constexpr int GX = RegShape<R>::GX[V - 1], GY = RegShape<R>::GY[V - 1];
+    constexpr int P = RegShape<R>::P[V - 1], T = RegShape<R>::T[V - 1], W = RegShape<R>::W[V - 1];
+    if constexpr (!reg_ok<R, N, GX, GY>()) {
+        return fail(CPWGPU_ERR_MODE, "plane variant %d is not built for N = %d", V, N);
+    } else {
+        const FftGeom &g = p->g;
+        const int S = reg_stride(g.nx);
+        const size_t dyn = plane_reg_lds<R, N, P>(S);
+        if (dyn > 160 * 1024)
+            return fail(CPWGPU_ERR_SIZE, "plane variant %d: %zu bytes of LDS", V, dyn);
+        static bool attr = false;
+        if (!attr) {
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_reg<R, N, GX, GY, P, T, W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = true;
+        }
+        ProfScope ps_(CPWGPU_K_PLANE, s);
+        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W>), dim3(((a.nb + P - 1) / P) * g.n[2]),
+                           dim3(T), dyn, s, colbuf, (const int *)p->col_uy.p, vr,
+                           (const double2 *)p->tw[0].p, g, a.nb, S);
+        HIPCHK(hipGetLastError());
+        return CPWGPU_OK;
+    }
+}
+
+template <typename R, int N> constexpr bool plane_var_ok(int v) {
+    return v == 0 || (v == 1 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>()) ||
+           (v == 2 && reg_ok<R, N, RegShape<R>::GX[1], RegShape<R>::GY[1]>());
+}
+
+// default plane variant of a mesh length
+template <typename R, int N> constexpr int plane_var_default() { return 0; }
+
+template <typename R, int N>
+int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
+    int v = p->plane_var[sizeof(R) == 4 ? 1 : 0];
+    if (v < 0 || !plane_var_ok<R, N>(v)) v = plane_var_default<R, N>();
+    if (v == 1) return launch_plane_reg<R, N, 1>(p, a, colbuf, vr, s);
+    if (v == 2) return launch_plane_reg<R, N, 2>(p, a, colbuf, vr, s);
+    return launch_fast<R, N>(p, a, colbuf, vr, ST_PLANE, s);
+}
+
 bool fast_sizes(const FftGeom &g) {
     bool col_ok = false, plane_ok = false;
 #define CPW_HAS(N) col_ok |= g.n[2] == N; plane_ok |= (g.n[0] == N && g.n[1] == N);
     CPW_FAST_SIZES(CPW_HAS)
 #undef CPW_HAS
     return col_ok && plane_ok;
+}
+
+// plane variant the next apply uses (defaults resolved), and whether v is
+// built for the plan's mesh
+template <typename R> int plane_var_resolve(const cpwgpu_plan_s *p, int v, bool *ok) {
+    const FftGeom &g = p->g;
+    *ok = v == 0 || v == -1;
+    if (!fast_sizes(g)) return 0;
+    int def = 0;
+#define CPW_PV(N)                                      \
+    if (g.n[0] == N) {                                 \
+        def = plane_var_default<R, N>();               \
+        if (v > 0) *ok = v < NPLANE_VAR && plane_var_ok<R, N>(v); \
+    }
+    CPW_FAST_SIZES(CPW_PV)
+#undef CPW_PV
+    return v < 0 || !*ok ? def : v;
 }
 
 // One stage of the local term: specialised kernels when the mesh allows,
@@ -388,7 +484,7 @@ int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
     if (fast_sizes(g)) {
         int st = CPWGPU_OK;
         if (stage == ST_PLANE) {
-#define CPW_PL(N) if (g.n[0] == N) st = launch_fast<R, N>(p, a, colbuf, vr, stage, s);
+#define CPW_PL(N) if (g.n[0] == N) st = launch_plane<R, N>(p, a, colbuf, vr, s);
             CPW_FAST_SIZES(CPW_PL)
 #undef CPW_PL
         } else {
@@ -451,41 +547,51 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
         anl = (const cplx<R> *)p->anl64.p;
     } else {
         if (!p->anl32_valid) {
-            if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda * nanl))) return st;
+            p->lda32 = (p->lda + 1) & ~1L;
+            if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda32 * nanl))) return st;
             const long tot = (long)g.mtxd * nanl;
             hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                                (const double2 *)p->anl64.p, p->lda, (cplx<float> *)p->anl32.p,
-                               p->lda, g.mtxd, nanl);
+                               p->lda32, g.mtxd, nanl);
             HIPCHK(hipGetLastError());
             p->anl32_valid = true;
         }
         anl = (const cplx<R> *)p->anl32.p;
     }
-    constexpr int BM = proj_bm<R>(), BN = proj_bn<R>();
+    const long lda = sizeof(R) == 8 ? p->lda : p->lda32;
+    const int ix = sizeof(R) == 4 ? 1 : 0;
+    const bool direct = p->proj_var[ix] == 1;
+    const int BM = direct ? direct_bm<R>() : proj_bm<R>(), BN = direct ? direct_bn<R>() : proj_bn<R>();
     const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
-    constexpr int BK = GemmCfg<R>::BK;
+    // K slices are whole LDS stages of k_proj / whole chunks of k_proj_direct
+    const int BK = direct ? 4 * DirectCfg<R>::KPL : GemmCfg<R>::BK;
     const int ksteps = (g.mtxd + BK - 1) / BK;
-    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+    const int target = p->splitk[ix] > 0 ? p->splitk[ix] : 256;
+    int nsl = std::max(1, std::min(ksteps, (target + tm * tn - 1) / (tm * tn)));
     const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
     nsl = (g.mtxd + kslice - 1) / kslice;
     if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
     if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
     {
         ProfScope ps_(CPWGPU_K_PROJ, s);
-        hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, p->lda,
-                           (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig, kslice);
+        if (direct)
+            hipLaunchKernelGGL(k_proj_direct<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
+                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig,
+                               kslice);
+        else
+            hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
+                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig,
+                               kslice, nullptr);
     }
-    const long tot = (long)nanl * neig;
     {
         ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
-        hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                           (const cplx<R> *)p->part.p, (const double *)p->xnl.p, (cplx<R> *)p->dhd.p,
-                           nanl, neig, nsl);
+        launch_proj_reduce<R>((const cplx<R> *)p->part.p, (const double *)p->xnl.p,
+                              (cplx<R> *)p->dhd.p, nanl, neig, nsl, s);
     }
     {
         ProfScope ps_(CPWGPU_K_BACK, s);
         hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
-                           dim3(GEMM_THREADS), 0, s, anl, p->lda, (const cplx<R> *)p->dhd.p,
+                           dim3(GEMM_THREADS), 0, s, anl, lda, (const cplx<R> *)p->dhd.p,
                            (double2 *)hpsi, ldh, g.mtxd, nanl, neig, beta);
     }
     HIPCHK(hipGetLastError());
@@ -659,7 +765,6 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
     const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
     nsl = (mtxd + kslice - 1) / kslice;
     if ((rc = g_ss.part.reserve(16 * (size_t)nsl * nb * nb))) return rc;
-    const long tot = (long)nb * nb;
     double2 *S = (double2 *)g_ss.s.p, *M = (double2 *)g_ss.m.p, *W = (double2 *)g_ss.w.p,
             *V = (double2 *)g_ss.v.p;
     for (int which = 0; which < 2; which++) {
@@ -668,9 +773,8 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
         hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
                            (const cplx<double> *)psi, ldp, y, ldy, (cplx<double> *)g_ss.part.p, mtxd,
                            nb, nb, kslice);
-        hipLaunchKernelGGL(k_proj_reduce<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                           (const cplx<double> *)g_ss.part.p, (const double *)nullptr,
-                           (cplx<double> *)(which == 0 ? S : M), nb, nb, nsl);
+        launch_proj_reduce<double>((const cplx<double> *)g_ss.part.p, nullptr,
+                                   (cplx<double> *)(which == 0 ? S : M), nb, nb, nsl, st);
     }
     int *dinfo = (int *)g_ss.info.p;
     hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(SS_THREADS), 0, st, S, nb, dinfo);
@@ -821,9 +925,8 @@ int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const 
             hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s,
                                (const cplx<double> *)p->anl64.p, p->lda, psi, ldp,
                                (cplx<double> *)p->part.p, g.mtxd, nanl, neig, kslice, w);
-            hipLaunchKernelGGL(k_proj_reduce<double>, dim3((unsigned)((mat + 255) / 256)), dim3(256), 0, s,
-                               (const cplx<double> *)p->part.p, (const double *)nullptr,
-                               (cplx<double> *)g_fw.p.p + (size_t)j * mat, nanl, neig, nsl);
+            launch_proj_reduce<double>((const cplx<double> *)p->part.p, nullptr,
+                                       (cplx<double> *)g_fw.p.p + (size_t)j * mat, nanl, neig, nsl, s);
         }
         hipLaunchKernelGGL(k_force_nl, dim3(natom), dim3(256), 0, s, (const double2 *)g_fw.p.p, nanl,
                            neig, (const double *)p->xnl.p, occ, (const int *)g_fw.patom.p,
@@ -859,6 +962,123 @@ int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const 
             force[3 * a + i] = v;
         }
     if (enl) *enl = e_nl;
+    return CPWGPU_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// ---- autotuner ---------------------------------------------------------------
+// Key of a tuned shape: the mesh, the x-box width, the number of z columns, the
+// projector count and the band count decide which kernels win.
+struct TuneKey {
+    int prec, n1, n2, n3, nx, ncol, nanl, neig;
+    bool operator==(const TuneKey &o) const {
+        return prec == o.prec && n1 == o.n1 && n2 == o.n2 && n3 == o.n3 && nx == o.nx &&
+               ncol == o.ncol && nanl == o.nanl && neig == o.neig;
+    }
+};
+
+bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *splitk) {
+    const char *path = std::getenv("CPWGPU_TUNE_CACHE");
+    if (!path || !*path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    TuneKey r;
+    int pv, jv, sk;
+    bool found = false;
+    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3, &r.nx,
+                       &r.ncol, &r.nanl, &r.neig, &pv, &jv, &sk) == 11) {
+        if (r == k) {  // the last entry for a key wins
+            *plane = pv;
+            *proj = jv;
+            *splitk = sk;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void tune_cache_store(const TuneKey &k, int plane, int proj, int splitk) {
+    const char *path = std::getenv("CPWGPU_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
+                 k.nanl, k.neig, plane, proj, splitk);
+    std::fclose(f);
+}
+
+// median-free timing: the best of `reps` applies after one warm-up
+int time_apply(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+               int prec, hipStream_t s, int reps, float *best) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+    *best = 3.4e38f;
+    for (int r = 0; r < reps && !st; r++) {
+        HIPCHK(hipEventRecord(a, s));
+        st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+        HIPCHK(hipEventRecord(b, s));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        *best = std::min(*best, ms);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return st;
+}
+
+int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+              int prec, hipStream_t s) {
+    const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
+    const FftGeom &g = p->g;
+    const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
+    int pv = -1, jv = -1, sk = -1;
+    if (tune_cache_lookup(key, &pv, &jv, &sk)) {
+        bool ok;
+        if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
+        else plane_var_resolve<double>(p, pv, &ok);
+        p->plane_var[ix] = ok ? pv : -1;
+        p->proj_var[ix] = jv == 0 || jv == 1 ? jv : -1;
+        p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
+        return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+    }
+    int st;
+    float best = 3.4e38f, t;
+    int best_pv = 0;
+    for (int v = 0; v < NPLANE_VAR; v++) {
+        bool ok;
+        if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, v, &ok);
+        else plane_var_resolve<double>(p, v, &ok);
+        if (!ok) continue;
+        p->plane_var[ix] = v;
+        if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+        if (t < best) { best = t; best_pv = v; }
+    }
+    p->plane_var[ix] = best_pv;
+    int best_sk = -1, best_jv = 0;
+    if (p->nanl > 0) {
+        // projection kernel x split-K target
+        const int cand[4] = {128, 256, 512, 1024};
+        best = 3.4e38f;
+        for (int jv = 0; jv < 2; jv++)
+            for (int c : cand) {
+                p->proj_var[ix] = jv;
+                p->splitk[ix] = c;
+                if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+                if (t < best) { best = t; best_sk = c; best_jv = jv; }
+            }
+    }
+    p->proj_var[ix] = best_jv;
+    p->splitk[ix] = best_sk;
+    // leave hpsi holding H psi of the chosen configuration
+    if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
+    tune_cache_store(key, best_pv, best_jv, best_sk);
     return CPWGPU_OK;
 }
 
@@ -1115,6 +1335,63 @@ int cpwgpu_profile_read(double *ms, long *launches, int n) {
     }
     g_prof.recs.clear();
     return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
+    if (what == CPWGPU_TUNE_PLANE) {
+        bool ok = true;
+        if (value >= 0) {
+            if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(plan, value, &ok);
+            else plane_var_resolve<double>(plan, value, &ok);
+        }
+        if (!ok) return fail(CPWGPU_ERR_MODE, "plane variant %d is not built for this mesh", value);
+        plan->plane_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_PROJ) {
+        if (value > 1) return fail(CPWGPU_ERR_MODE, "projection kernel %d", value);
+        plan->proj_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_SPLITK) {
+        if (value >= 0 && (value < 64 || value > 4096))
+            return fail(CPWGPU_ERR_ARG, "split-K target %d outside 64..4096", value);
+        plan->splitk[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
+    return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);
+}
+
+int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) {
+    if (!plan || !value) return fail(CPWGPU_ERR_ARG, "null argument");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
+    bool ok;
+    if (what == CPWGPU_TUNE_PLANE) {
+        *value = prec == CPWGPU_PREC_FP32 ? plane_var_resolve<float>(plan, plan->plane_var[ix], &ok)
+                                          : plane_var_resolve<double>(plan, plan->plane_var[ix], &ok);
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_SPLITK) {
+        *value = plan->splitk[ix] > 0 ? plan->splitk[ix] : 256;
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_PROJ) {
+        *value = plan->proj_var[ix] == 1 ? 1 : 0;
+        return CPWGPU_OK;
+    }
+    return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);
+}
+
+int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                     long ldh, int prec, void *stream) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    if (neig <= 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
+    return tune_plan(plan, neig, psi_dev, ldp, hpsi_dev, ldh, prec, (hipStream_t)stream);
 }
 
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out) {

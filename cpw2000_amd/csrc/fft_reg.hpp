@@ -1,0 +1,244 @@
+// Register-resident plane transforms: each line of length N = G*M lives in the
+// registers of G adjacent lanes (M values per lane).  A transform is an
+// in-register DFT_M, a lane-dependent twiddle and a DFT_G across the G lanes
+// done with DPP butterflies (no LDS, no barrier).  Used for the y and x passes
+// of the per-(z, band) plane stage of H|psi>: only the y/x-box sub-plane is
+// kept in LDS, so more blocks fit per CU than with LDS-exchanged passes.
+//
+// Layouts, for a line f[0..N) and the lane at position g < G of its group:
+//   DIF  in : v[j]  = f[g + G j]           out: v[k1] = F[k1 + M bitrev(g)]
+//   DIT  in : v[k1] = f[k1 + M bitrev(g)]  out: v[m]  = F[g + G m]
+// so a DIT transform consumes the DIF output directly, and the DIF output
+// gives each lane M consecutive points (contiguous V(r) loads).
+//
+// DPP patterns: position xor 1 and xor 2 are quad permutations; position xor 4
+// (G = 8) uses row_half_mirror (lane i <-> 7 - i within 8 lanes) with the lane
+// to position map g = L for L < 4, g = 11 - L for L >= 4, under which the
+// mirror pairs are exactly the positions g, g ^ 4, and xor 1 / xor 2 of the
+// position remain xor 1 / xor 2 of the lane.
+#pragma once
+
+#include "fft_fast.hpp"
+
+namespace cpwgpu {
+
+template <int H> constexpr int dpp_ctrl() { return H == 1 ? 0xB1 : H == 2 ? 0x4E : 0x141; }
+
+template <int CTRL> __device__ __forceinline__ float xlane(float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double xlane(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi =
+        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+constexpr int ilog2(int g) { return g <= 1 ? 0 : 1 + ilog2(g / 2); }
+constexpr int bitrev(int v, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+// Per-lane constants of a G-lane group: position, butterfly signs and the
+// upper-half twiddles W_{2h}^{(g mod 2h) - h} (backward sign) of each stage.
+template <typename R, int G> struct Lanes {
+    int g;    // position in the group
+    int k2;   // bitrev(g): DIF output / DIT input block index
+    R s1, s2, s4;
+    cplx<R> w2, w4;
+    __device__ __forceinline__ Lanes() {
+        const int l = (int)(threadIdx.x % G);
+        g = (G == 8 && l >= 4) ? 11 - l : l;
+        k2 = 0;
+        for (int i = 0; i < ilog2(G); i++) k2 |= ((g >> i) & 1) << (ilog2(G) - 1 - i);
+        s1 = (g & 1) ? (R)-1 : (R)1;
+        s2 = (g & 2) ? (R)-1 : (R)1;
+        s4 = (g & 4) ? (R)-1 : (R)1;
+        // W_4^q, q = (g mod 4) - 2 in {0, 1} for the upper half of a 4-block
+        w2 = ((g & 3) == 3) ? mk<R>(0, 1) : mk<R>(1, 0);
+        // W_8^q, q = (g mod 8) - 4 in {0..3} for the upper half of an 8-block
+        const R h = (R)0.70710678118654752440;
+        const int q = g - 4;
+        w4 = q == 1 ? mk<R>(h, h) : q == 2 ? mk<R>(0, 1) : q == 3 ? mk<R>(-h, h) : mk<R>(1, 0);
+    }
+};
+
+// A value the compiler cannot see through: twiddle loads indexed by it stay
+// inside the loop that uses them (LDS reads) instead of being hoisted and held
+// in registers for the whole kernel.
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+template <int H, typename R> __device__ __forceinline__ cplx<R> bfly(cplx<R> m, R s) {
+    constexpr int C = dpp_ctrl<H>();
+    return mk<R>(xlane<C>(m.x) + s * m.x, xlane<C>(m.y) + s * m.y);  // partner + s * mine
+}
+template <int DIR, typename R> __device__ __forceinline__ cplx<R> cmul_dir(cplx<R> a, cplx<R> w) {
+    return cmul(a, DIR > 0 ? w : mk<R>(w.x, -w.y));
+}
+
+// DIF transform of a G-lane line (layouts above); twN[e] = exp(+2 pi i e/N).
+template <int DIR, int G, int M, typename R>
+__device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
+                                         const cplx<R> *twN) {
+    dft<M, DIR, R>(v);
+    if constexpr (G > 1) {
+        const int gi = opaque(L.g);
+#pragma unroll
+        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], twN[gi * k]);
+        if constexpr (G >= 8) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<4>(v[k], L.s4), L.w4);
+        }
+        if constexpr (G >= 4) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<2>(v[k], L.s2), L.w2);
+        }
+#pragma unroll
+        for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
+    }
+}
+
+// DIT transform of a G-lane line, the transpose of line_dif's structure.
+template <int DIR, int G, int M, typename R>
+__device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
+                                         const cplx<R> *twN) {
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
+        if constexpr (G >= 4) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly<2>(cmul_dir<DIR>(v[k], L.w2), L.s2);
+        }
+        if constexpr (G >= 8) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly<4>(cmul_dir<DIR>(v[k], L.w4), L.s4);
+        }
+        const int gi = opaque(L.g);
+#pragma unroll
+        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], twN[gi * k]);
+    }
+    dft<M, DIR, R>(v);
+}
+
+// ---------------------------------------------------------------------------
+// plane stage, register variant: a block takes P planes (bands b0..b0+P-1 at
+// one z) with T threads and a register budget for W waves per SIMD.  LDS:
+// sub[P][N][S] (y rows of the x-box, stride S >= nx) and the twiddle table
+// twN[N].  Phases (barrier between each):
+//   gather columns -> inverse y (GY-lane lines over u) -> per row y: inverse
+//   x, V(r), forward x (GX-lane lines, no barrier) -> forward y -> scatter.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int GX, int GY, int P, int T, int W>
+__global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf,
+                                                 const int *__restrict__ col_uy,
+                                                 const R *__restrict__ vr,
+                                                 const double2 *__restrict__ twg, FftGeom g,
+                                                 int nb, int S) {
+    static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
+    constexpr int MX = N / GX, MY = N / GY;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<R> *twN = reinterpret_cast<cplx<R> *>(smem_raw);  // [N]
+    cplx<R> *sub = twN + N;                                 // [P][N][S]
+    const int n3 = g.n[2];
+    const int z = blockIdx.x % n3;
+    const int b0 = (blockIdx.x / n3) * P;
+    const int np = min(P, nb - b0);  // planes of this block that exist
+    const int nx = g.nx, kd1 = g.kd[0], ushift = N - nx, xhi = N - kd1;
+    const int tid = threadIdx.x;
+
+    load_twiddles(twN, twg, N);
+    for (int q = tid; q < P * N * S; q += T) sub[q] = mk<R>(0, 0);
+    __syncthreads();
+    for (int p = 0; p < np; p++) {
+        const cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
+        cplx<R> *sp = sub + p * N * S;
+        for (int c = tid; c < g.ncol; c += T) {
+            const int uy = col_uy[c];
+            sp[(uy >> 16) * S + (uy & 0xffff)] = cb[c];
+        }
+    }
+    __syncthreads();
+
+    {  // inverse y on the P * nx box lines
+        const Lanes<R, GY> L;
+        const int ntask = P * nx;
+        for (int task = tid / GY; task < ntask; task += T / GY) {
+            const int p = task / nx, u = task - p * nx;
+            cplx<R> *col = sub + p * N * S + u;
+            cplx<R> v[MY];
+#pragma unroll
+            for (int j = 0; j < MY; j++) v[j] = col[(L.g + GY * j) * S];
+            line_dif<+1, GY, MY>(v, L, twN);
+#pragma unroll
+            for (int k = 0; k < MY; k++) col[(k + MY * L.k2) * S] = v[k];
+        }
+    }
+    __syncthreads();
+
+    {  // rows: inverse x, V(r), forward x
+        const Lanes<R, GX> L;
+        const int ntask = P * N;
+        for (int task = tid / GX; task < ntask; task += T / GX) {
+            const int p = task / N, y = task - p * N;
+            cplx<R> *row = sub + (p * N + y) * S;
+            cplx<R> v[MX];
+#pragma unroll
+            for (int j = 0; j < MX; j++) {
+                const int x = L.g + GX * j;
+                v[j] = mk<R>(0, 0);
+                if (x <= kd1) v[j] = row[x];
+                else if (x >= xhi) v[j] = row[x - ushift];
+            }
+            line_dif<+1, GX, MX>(v, L, twN);
+            const R *vy = vr + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
+#pragma unroll
+            for (int k = 0; k < MX; k++) v[k] = scale(v[k], vy[k]);
+            line_dit<-1, GX, MX>(v, L, twN);
+#pragma unroll
+            for (int m = 0; m < MX; m++) {
+                const int x = L.g + GX * m;
+                if (x <= kd1) row[x] = v[m];
+                else if (x >= xhi) row[x - ushift] = v[m];
+            }
+        }
+    }
+    __syncthreads();
+
+    {  // forward y
+        const Lanes<R, GY> L;
+        const int ntask = P * nx;
+        for (int task = tid / GY; task < ntask; task += T / GY) {
+            const int p = task / nx, u = task - p * nx;
+            cplx<R> *col = sub + p * N * S + u;
+            cplx<R> v[MY];
+#pragma unroll
+            for (int k = 0; k < MY; k++) v[k] = col[(k + MY * L.k2) * S];
+            line_dit<-1, GY, MY>(v, L, twN);
+#pragma unroll
+            for (int m = 0; m < MY; m++) col[(L.g + GY * m) * S] = v[m];
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < np; p++) {
+        cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
+        const cplx<R> *sp = sub + p * N * S;
+        for (int c = tid; c < g.ncol; c += T) {
+            const int uy = col_uy[c];
+            cb[c] = sp[(uy >> 16) * S + (uy & 0xffff)];
+        }
+    }
+}
+
+// LDS bytes of k_plane_reg
+template <typename R, int N, int P> constexpr size_t plane_reg_lds(int S) {
+    return sizeof(cplx<R>) * ((size_t)N + (size_t)P * N * S);
+}
+
+}  // namespace cpwgpu

@@ -199,22 +199,217 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
         }
 }
 
-// dhd[n][a] = xnlkb[a] * sum_s part[s][n][a]   (fixed summation order: deterministic)
-template <typename R>
-__global__ void k_proj_reduce(const cplx<R> *__restrict__ part, const double *__restrict__ xnl,
-                              cplx<R> *__restrict__ dhd, int nanl, int neig, int nslice) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long tot = (long)nanl * neig;
-    if (t >= tot) return;
-    const int a = (int)(t % nanl);
-    R sr = 0, si = 0;
-    for (int s = 0; s < nslice; s++) {
-        cplx<R> v = part[(long)s * tot + t];
-        sr += v.x;
-        si += v.y;
+// ---------------------------------------------------------------------------
+// k_proj_direct: the product and partial layout of k_proj, without LDS.  Each
+// wave loads its own A (projector rows) and B (psi columns) fragments from
+// global memory into registers, one K chunk of 4*KPL ahead of the MFMAs, so
+// no barrier ever stalls the matrix cores; the waves of a block that share
+// tiles hit the same L1/L2 lines.  Requires anl rows 16-byte aligned (lda
+// even for complex64).  Full chunks use 16-byte vector loads; the slice tail
+// falls back to clamped, zero-masked element loads.
+// ---------------------------------------------------------------------------
+// wave tile (WM*16) x (WN*16), KPL K values per lane per chunk; sized so that
+// the accumulators and two chunks of loads stay within 256 registers
+template <typename R> struct DirectCfg;
+template <> struct DirectCfg<float> { static constexpr int WM = 4, WN = 2, KPL = 4; };
+template <> struct DirectCfg<double> { static constexpr int WM = 2, WN = 2, KPL = 4; };
+template <typename R> constexpr int direct_bm() { return 2 * DirectCfg<R>::WM * 16; }
+template <typename R> constexpr int direct_bn() { return 2 * DirectCfg<R>::WN * 16; }
+
+template <typename R, int KPL>
+__device__ __forceinline__ void load_row_chunk(cplx<R> (&dst)[KPL], const cplx<R> *__restrict__ row,
+                                               int i0, int ke, int mtxd, bool rowok) {
+    if (rowok && i0 + KPL <= ke) {
+        if constexpr (sizeof(R) == 4) {
+#pragma unroll
+            for (int q = 0; q < KPL / 2; q++) {
+                const float4 f = *reinterpret_cast<const float4 *>(row + i0 + 2 * q);
+                dst[2 * q] = mk<R>(f.x, f.y);
+                dst[2 * q + 1] = mk<R>(f.z, f.w);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < KPL; q++) dst[q] = row[i0 + q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < KPL; q++) {
+            const int i = i0 + q;
+            const cplx<R> v = row[min(i, mtxd - 1)];
+            const bool ok = rowok && i < ke;
+            dst[q] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+        }
     }
-    const R x = xnl ? (R)xnl[a] : (R)1;  // no scaling for plain Gram matrices
-    dhd[t] = mk<R>(x * sr, x * si);
+}
+
+template <int KPL>
+__device__ __forceinline__ void load_psi_chunk(double2 (&dst)[KPL], const double2 *__restrict__ col,
+                                               int i0, int ke, int mtxd, bool colok) {
+    if (colok && i0 + KPL <= ke) {
+#pragma unroll
+        for (int q = 0; q < KPL; q++) dst[q] = col[i0 + q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < KPL; q++) {
+            const int i = i0 + q;
+            const double2 v = col[min(i, mtxd - 1)];
+            const bool ok = colok && i < ke;
+            dst[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void k_proj_direct(const cplx<R> *__restrict__ anl,
+                                                              long lda,
+                                                              const double2 *__restrict__ psi,
+                                                              long ldp, cplx<R> *__restrict__ part,
+                                                              int mtxd, int nanl, int neig,
+                                                              int kslice) {
+    typedef typename MF<R>::v4 v4;
+    constexpr int WM = DirectCfg<R>::WM, WN = DirectCfg<R>::WN;
+    constexpr int KPL = DirectCfg<R>::KPL, KC = 4 * KPL;
+    constexpr int BM = direct_bm<R>(), BN = direct_bn<R>();
+    const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int lr = lane & 15, lg = lane >> 4;
+    const cplx<R> *arow[WM];
+    bool aok[WM];
+#pragma unroll
+    for (int t = 0; t < WM; t++) {
+        const int a = a0 + wm * WM * 16 + t * 16 + lr;
+        aok[t] = a < nanl;
+        arow[t] = anl + (long)min(a, nanl - 1) * lda;
+    }
+    const double2 *bcol[WN];
+    bool bok[WN];
+#pragma unroll
+    for (int t = 0; t < WN; t++) {
+        const int n = n0 + wn * WN * 16 + t * 16 + lr;
+        bok[t] = n < neig;
+        bcol[t] = psi + (long)min(n, neig - 1) * ldp;
+    }
+    v4 P1[WM][WN], P2[WM][WN], P3[WM][WN];
+#pragma unroll
+    for (int i = 0; i < WM; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++) {
+            P1[i][j] = v4{0, 0, 0, 0};
+            P2[i][j] = v4{0, 0, 0, 0};
+            P3[i][j] = v4{0, 0, 0, 0};
+        }
+    cplx<R> ra0[WM][KPL], ra1[WM][KPL];
+    double2 rb0[WN][KPL], rb1[WN][KPL];
+    auto load = [&](cplx<R>(&ra)[WM][KPL], double2(&rb)[WN][KPL], int k0) {
+        const int i0 = k0 + lg * KPL;
+#pragma unroll
+        for (int t = 0; t < WM; t++) load_row_chunk<R, KPL>(ra[t], arow[t], i0, ke, mtxd, aok[t]);
+#pragma unroll
+        for (int t = 0; t < WN; t++) load_psi_chunk<KPL>(rb[t], bcol[t], i0, ke, mtxd, bok[t]);
+    };
+    auto compute = [&](const cplx<R>(&ra)[WM][KPL], const double2(&rb)[WN][KPL]) {
+#pragma unroll
+        for (int u = 0; u < KPL; u++) {
+            R ar[WM], ai[WM], as[WM], br[WN], bi[WN], bs[WN];
+#pragma unroll
+            for (int t = 0; t < WM; t++) {
+                ar[t] = ra[t][u].x;
+                ai[t] = ra[t][u].y;
+                as[t] = ar[t] - ai[t];
+            }
+#pragma unroll
+            for (int t = 0; t < WN; t++) {
+                br[t] = (R)rb[t][u].x;
+                bi[t] = (R)rb[t][u].y;
+                bs[t] = br[t] + bi[t];
+            }
+#pragma unroll
+            for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+                for (int ni = 0; ni < WN; ni++) {
+                    P1[mi][ni] = MF<R>::mma(ar[mi], br[ni], P1[mi][ni]);
+                    P2[mi][ni] = MF<R>::mma(ai[mi], bi[ni], P2[mi][ni]);
+                    P3[mi][ni] = MF<R>::mma(as[mi], bs[ni], P3[mi][ni]);
+                }
+        }
+    };
+    if (kb < ke) load(ra0, rb0, kb);
+    for (int k0 = kb; k0 < ke; k0 += 2 * KC) {
+        const bool more1 = k0 + KC < ke, more2 = k0 + 2 * KC < ke;
+        if (more1) load(ra1, rb1, k0 + KC);
+        compute(ra0, rb0);
+        if (more2) load(ra0, rb0, k0 + 2 * KC);
+        if (more1) compute(ra1, rb1);
+    }
+    cplx<R> *ps = part + (long)blockIdx.z * neig * nanl;
+#pragma unroll
+    for (int mi = 0; mi < WM; mi++)
+#pragma unroll
+        for (int ni = 0; ni < WN; ni++) {
+            const int n = n0 + wn * WN * 16 + ni * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int a = a0 + wm * WM * 16 + mi * 16 + MF<R>::row(lg, r);
+                const R p1 = P1[mi][ni][r], p2 = P2[mi][ni][r], p3 = P3[mi][ni][r];
+                if (a < nanl && n < neig) ps[(long)n * nanl + a] = mk<R>(p1 + p2, p3 - p1 + p2);
+            }
+        }
+}
+
+// dhd[n][a] = xnlkb[a] * sum_s part[s][n][a]   (fixed summation order: deterministic)
+// Block of 256 threads = 64 consecutive outputs x 4 slice groups; group g sums
+// slices g, g+4, ... into four independent partial sums (16 loads in flight
+// per thread), then the groups are combined through LDS in group order.
+constexpr int RED_OUT = 64, RED_GRP = 4;
+template <typename R>
+__global__ __launch_bounds__(RED_OUT * RED_GRP) void k_proj_reduce(const cplx<R> *__restrict__ part,
+                                                                  const double *__restrict__ xnl,
+                                                                  cplx<R> *__restrict__ dhd,
+                                                                  int nanl, int neig, int nslice) {
+    __shared__ cplx<R> red[RED_GRP][RED_OUT];
+    const int o = threadIdx.x % RED_OUT, grp = threadIdx.x / RED_OUT;
+    const long tot = (long)nanl * neig;
+    const long t = (long)blockIdx.x * RED_OUT + o;
+    const long tc = t < tot ? t : tot - 1;  // clamped, always a valid address
+    R sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+    int s = grp;
+    for (; s + 3 * RED_GRP < nslice; s += 4 * RED_GRP) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const cplx<R> v = part[(long)(s + u * RED_GRP) * tot + tc];
+            sr[u] += v.x;
+            si[u] += v.y;
+        }
+    }
+    for (; s < nslice; s += RED_GRP) {
+        const cplx<R> v = part[(long)s * tot + tc];
+        sr[0] += v.x;
+        si[0] += v.y;
+    }
+    red[grp][o] = mk<R>((sr[0] + sr[1]) + (sr[2] + sr[3]), (si[0] + si[1]) + (si[2] + si[3]));
+    __syncthreads();
+    if (grp == 0 && t < tot) {
+        R r = 0, i = 0;
+#pragma unroll
+        for (int g2 = 0; g2 < RED_GRP; g2++) {
+            r += red[g2][o].x;
+            i += red[g2][o].y;
+        }
+        const R x = xnl ? (R)xnl[t % nanl] : (R)1;  // no scaling for plain Gram matrices
+        dhd[t] = mk<R>(x * r, x * i);
+    }
+}
+
+// Launches k_proj_reduce over nanl * neig outputs (nothing to do when empty).
+template <typename R>
+inline void launch_proj_reduce(const cplx<R> *part, const double *xnl, cplx<R> *dhd, int nanl,
+                               int neig, int nslice, hipStream_t s) {
+    const long tot = (long)nanl * neig;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + RED_OUT - 1) / RED_OUT)),
+                       dim3(RED_OUT * RED_GRP), 0, s, part, xnl, dhd, nanl, neig, nslice);
 }
 
 constexpr int BP_BI = 64;  // plane waves per block (16 per wave)

@@ -47,7 +47,11 @@ EXPORTS = (
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
     "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz", "cpwgpu_plan_forces",
     "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
+    "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune",
 )
+
+# tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ = 0, 1, 2
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
@@ -96,6 +100,9 @@ def lib():
         L.cpwgpu_graph_create.argtypes = [C.POINTER(vp), C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
         L.cpwgpu_graph_launch.argtypes = [vp, vp]
         L.cpwgpu_graph_destroy.argtypes = [vp]
+        L.cpwgpu_plan_set_variant.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.cpwgpu_plan_get_variant.argtypes = [vp, C.c_int, C.c_int, ip]
+        L.cpwgpu_plan_tune.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
         _lib = L
     return _lib
@@ -236,6 +243,33 @@ class Plan:
         _check(lib().cpwgpu_plan_info(self._h, out))
         keys = ("mtxd", "n1", "n2", "n3", "id", "kd1", "kd2", "kd3", "ncol", "nanl")
         return dict(zip(keys, list(out)))
+
+    def set_variant(self, prec, what, value):
+        """Kernel choice for a precision: what = TUNE_PLANE (plane-stage kernel,
+        0 LDS two-pass, 1/2 register shapes, -1 default), TUNE_PROJ
+        (projection GEMM, 0 LDS-staged, 1 direct loads) or TUNE_SPLITK
+        (projection GEMM split-K target blocks, -1 default)."""
+        _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
+
+    def get_variant(self, prec, what):
+        out = C.c_int(0)
+        _check(lib().cpwgpu_plan_get_variant(self._h, int(prec), int(what), C.byref(out)))
+        return out.value
+
+    def tune(self, psi_t, hpsi_t, prec=PREC_FP32, stream=None):
+        """Autotune the plan's kernels on these device arrays (hpsi_t receives
+        H psi); returns {"plane": variant, "splitk": target}."""
+        import torch
+
+        neig = psi_t.shape[0]
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        _check(lib().cpwgpu_plan_tune(self._h, int(neig), C.c_void_p(psi_t.data_ptr()), _ld(psi_t),
+                                      C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t), int(prec),
+                                      C.c_void_p(stream)))
+        return {"plane": self.get_variant(prec, TUNE_PLANE),
+                "proj": self.get_variant(prec, TUNE_PROJ),
+                "splitk": self.get_variant(prec, TUNE_SPLITK)}
 
     def set_potential(self, vscr):
         v = np.ascontiguousarray(vscr, np.float64)

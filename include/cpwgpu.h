@@ -104,6 +104,31 @@ int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long 
 /* Sizes of the plan: out[0..9] = mtxd, n1, n2, n3, id, kd1, kd2, kd3, ncol, nanl. */
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 
+/* ---- kernel variants and autotuning (no counterpart in the reference) ------- */
+
+/* Tunable choices of a plan, per precision (CPWGPU_PREC_*):
+ *   CPWGPU_TUNE_PLANE   kernel of the y/x plane stage: 0 = LDS two-pass lines,
+ *                       1, 2 = register-resident lines (DPP butterflies) in two
+ *                       shapes; -1 = the default for the mesh
+ *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
+ *                       GEMM (64..4096); -1 = default (256)
+ *   CPWGPU_TUNE_PROJ    kernel of the projection GEMM: 0 = LDS-staged tiles,
+ *                       1 = direct register loads; -1 = default (0) */
+#define CPWGPU_TUNE_PLANE 0
+#define CPWGPU_TUNE_SPLITK 1
+#define CPWGPU_TUNE_PROJ 2
+int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
+/* Current choice (the value used by the next apply, defaults resolved). */
+int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);
+/* Autotuner: times cpwgpu_plan_apply for every plane variant built for the
+ * mesh, then both projection kernels with a few split-K targets, on the
+ * caller's arrays (neig bands, stream), and keeps the fastest in the plan.  hpsi_dev is overwritten with H psi.
+ * If the environment variable CPWGPU_TUNE_CACHE names a file, choices are
+ * read from it for meshes already tuned (no timing) and appended after
+ * tuning; the key is (precision, n1, n2, n3, nx, ncol, nanl, neig). */
+int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                     long ldh, int prec, void *stream);
+
 /* ---- several k-points in one graph launch -------------------------------------- */
 
 typedef struct cpwgpu_graph_s *cpwgpu_graph_t;
