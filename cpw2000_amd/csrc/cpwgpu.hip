@@ -185,6 +185,9 @@ struct cpwgpu_plan_s {
     // projection GEMM kernel per precision: 0 LDS-staged k_proj, 1 k_proj_direct
     // (-1 = default)
     int proj_var[2] = {-1, -1};
+    // back-projection GEMM kernel per precision: 0 LDS-staged k_back, 1
+    // k_back_direct (-1 = default)
+    int back_var[2] = {-1, -1};
     // workspaces
     DevBuf colbuf, part, dhd;
     DevBuf gw;                // g_j of each plane wave as doubles [3][mtxd] (force projections)
@@ -343,7 +346,7 @@ template <typename R, int N>
 int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, int stage,
                 hipStream_t s) {
     const FftGeom &g = p->g;
-    constexpr int CT = FT / (N / split_a(N));
+    constexpr int CT = col_tile<N>();
     const int ntiles = (g.ncol + CT - 1) / CT;
     const dim3 grid(8 * ((a.nb + 7) / 8) * ntiles);
     if (stage == ST_COL_BWD) {
@@ -360,7 +363,7 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
                            p->inv_n, a.acc);
     } else {
         const size_t dyn =
-            sizeof(cplx<R>) * ((size_t)N * (g.nx | 1) + (size_t)plane_rows_fast<N>() * row_stride<N>());
+            sizeof(cplx<R>) * ((size_t)N * plane_sub_stride(g.nx) + (size_t)plane_rows_fast<N>() * row_stride<N>());
         const size_t stat = sizeof(cplx<R>) * 2 * N;
         if (dyn + stat > 160 * 1024)
             return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS", N, N);
@@ -381,18 +384,18 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
 }
 
 // ---- plane stage variants ---------------------------------------------------
-// 0: k_plane_fast (LDS two-pass lines); 1, 2: k_plane_reg shapes
+// 0: k_plane_fast (LDS two-pass lines); 1..3: k_plane_reg shapes
 // (x lanes GX, y lanes GY, planes per block P, threads T) per precision.
 template <typename R> struct RegShape;
 template <> struct RegShape<float> {
-    static constexpr int GX[2] = {2, 4}, GY[2] = {4, 4}, P[2] = {2, 2}, T[2] = {320, 384};
-    static constexpr int W[2] = {3, 4};
+    static constexpr int GX[3] = {2, 4, 4}, GY[3] = {4, 4, 4}, P[3] = {2, 2, 1};
+    static constexpr int T[3] = {320, 384, 320}, W[3] = {3, 4, 5};
 };
 template <> struct RegShape<double> {
-    static constexpr int GX[2] = {8, 8}, GY[2] = {8, 8}, P[2] = {1, 1}, T[2] = {256, 384};
-    static constexpr int W[2] = {3, 5};
+    static constexpr int GX[3] = {8, 8, 4}, GY[3] = {8, 8, 8}, P[3] = {1, 1, 1};
+    static constexpr int T[3] = {256, 320, 256}, W[3] = {3, 4, 3};
 };
-constexpr int NPLANE_VAR = 3;
+constexpr int NPLANE_VAR = 4;
 
 // a register shape is built for a mesh length when both splits divide N and a
 // lane holds at most 288 bytes of one line
@@ -402,7 +405,7 @@ template <typename R, int N, int GX, int GY> constexpr bool reg_ok() {
 }
 
 // LDS row stride of the register variant's sub-plane for nx box columns
-inline int reg_stride(int nx) { return nx | 1; }
+inline int reg_stride(int nx) { return plane_sub_stride(nx); }
 
 template <typename R, int N, int V>
 int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr,
@@ -434,7 +437,8 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
 
 template <typename R, int N> constexpr bool plane_var_ok(int v) {
     return v == 0 || (v == 1 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>()) ||
-           (v == 2 && reg_ok<R, N, RegShape<R>::GX[1], RegShape<R>::GY[1]>());
+           (v == 2 && reg_ok<R, N, RegShape<R>::GX[1], RegShape<R>::GY[1]>()) ||
+           (v == 3 && reg_ok<R, N, RegShape<R>::GX[2], RegShape<R>::GY[2]>());
 }
 
 // default plane variant of a mesh length
@@ -446,6 +450,7 @@ int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R 
     if (v < 0 || !plane_var_ok<R, N>(v)) v = plane_var_default<R, N>();
     if (v == 1) return launch_plane_reg<R, N, 1>(p, a, colbuf, vr, s);
     if (v == 2) return launch_plane_reg<R, N, 2>(p, a, colbuf, vr, s);
+    if (v == 3) return launch_plane_reg<R, N, 3>(p, a, colbuf, vr, s);
     return launch_fast<R, N>(p, a, colbuf, vr, ST_PLANE, s);
 }
 
@@ -590,9 +595,15 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
     }
     {
         ProfScope ps_(CPWGPU_K_BACK, s);
-        hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
-                           dim3(GEMM_THREADS), 0, s, anl, lda, (const cplx<R> *)p->dhd.p,
-                           (double2 *)hpsi, ldh, g.mtxd, nanl, neig, beta);
+        const dim3 grid((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN);
+        if (p->back_var[ix] == 1)
+            hipLaunchKernelGGL(k_back_direct<R>, grid, dim3(GEMM_THREADS), 0, s, anl, lda,
+                               (const cplx<R> *)p->dhd.p, (double2 *)hpsi, ldh, g.mtxd, nanl, neig,
+                               beta);
+        else
+            hipLaunchKernelGGL(k_back<R>, grid, dim3(GEMM_THREADS), 0, s, anl, lda,
+                               (const cplx<R> *)p->dhd.p, (double2 *)hpsi, ldh, g.mtxd, nanl, neig,
+                               beta);
     }
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
@@ -980,19 +991,20 @@ struct TuneKey {
     }
 };
 
-bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *splitk) {
+bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
     TuneKey r;
-    int pv, jv, sk;
+    int pv, jv, bv, sk;
     bool found = false;
-    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3, &r.nx,
-                       &r.ncol, &r.nanl, &r.neig, &pv, &jv, &sk) == 11) {
+    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3, &r.nx,
+                       &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk) == 12) {
         if (r == k) {  // the last entry for a key wins
             *plane = pv;
             *proj = jv;
+            *back = bv;
             *splitk = sk;
             found = true;
         }
@@ -1001,13 +1013,13 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *splitk) {
     return found;
 }
 
-void tune_cache_store(const TuneKey &k, int plane, int proj, int splitk) {
+void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
-                 k.nanl, k.neig, plane, proj, splitk);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
+                 k.nanl, k.neig, plane, proj, back, splitk);
     std::fclose(f);
 }
 
@@ -1038,13 +1050,14 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
     const FftGeom &g = p->g;
     const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
-    int pv = -1, jv = -1, sk = -1;
-    if (tune_cache_lookup(key, &pv, &jv, &sk)) {
+    int pv = -1, jv = -1, bv = -1, sk = -1;
+    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk)) {
         bool ok;
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
         p->plane_var[ix] = ok ? pv : -1;
         p->proj_var[ix] = jv == 0 || jv == 1 ? jv : -1;
+        p->back_var[ix] = bv == 0 || bv == 1 ? bv : -1;
         p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
         return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
     }
@@ -1076,9 +1089,19 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     }
     p->proj_var[ix] = best_jv;
     p->splitk[ix] = best_sk;
+    int best_bv = 0;
+    if (p->nanl > 0) {
+        best = 3.4e38f;
+        for (int bv = 0; bv < 2; bv++) {
+            p->back_var[ix] = bv;
+            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+            if (t < best) { best = t; best_bv = bv; }
+        }
+    }
+    p->back_var[ix] = best_bv;
     // leave hpsi holding H psi of the chosen configuration
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
-    tune_cache_store(key, best_pv, best_jv, best_sk);
+    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk);
     return CPWGPU_OK;
 }
 
@@ -1356,6 +1379,11 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         plan->proj_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
+    if (what == CPWGPU_TUNE_BACK) {
+        if (value > 1) return fail(CPWGPU_ERR_MODE, "back-projection kernel %d", value);
+        plan->back_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
     if (what == CPWGPU_TUNE_SPLITK) {
         if (value >= 0 && (value < 64 || value > 4096))
             return fail(CPWGPU_ERR_ARG, "split-K target %d outside 64..4096", value);
@@ -1381,6 +1409,10 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
     }
     if (what == CPWGPU_TUNE_PROJ) {
         *value = plan->proj_var[ix] == 1 ? 1 : 0;
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_BACK) {
+        *value = plan->back_var[ix] == 1 ? 1 : 0;
         return CPWGPU_OK;
     }
     return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);

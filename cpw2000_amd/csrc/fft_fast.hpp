@@ -23,18 +23,21 @@ __device__ __forceinline__ bool xcd_band_tile(int b, int ntiles, int nb, int &ba
 }
 
 // Two-pass transform of nl lines in LDS: line l at buf + l*ldist, element j at
-// + j*estr.  Lines go in rounds of FT/B whole lines (all B work items of a
-// line in the same round).  Threads are numbered line fastest (work item t of
-// round line r is thread t*LPR + r): 16 neighbouring lanes touch the same
-// position of 16 neighbouring lines, which the callers lay out at a stride
-// that is odd in 16-byte units, so every LDS access is free of bank conflicts.
-// The result is stored back in natural order.
+// + j*estr.  Lines go in rounds of whole lines (all B work items of a line in
+// the same round), LPW = 64/B lines per wave: in wave w, lane t*LPW + i is work
+// item t of line w*LPW + i, so neighbouring lanes touch the same position of
+// neighbouring lines, and a round that is not full occupies the first waves
+// only (the others skip it).  The result is stored back in natural order.
+template <int N> constexpr int lines_per_wave() { return 64 / (N / split_a(N)); }
+
 template <typename R, int N, int DIR>
 __device__ __forceinline__ void line_fft(cplx<R> *buf, int nl, int ldist, int estr,
                                          const cplx<R> *tw) {
     constexpr int A = split_a(N), B = N / A;
-    constexpr int LPR = FT / B;
-    const int r = (int)threadIdx.x % LPR, t = (int)threadIdx.x / LPR;
+    constexpr int LPW = lines_per_wave<N>();
+    constexpr int LPR = (FT / 64) * LPW;  // lines per round
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int t = lane / LPW, r = wave * LPW + lane - (lane / LPW) * LPW;
     for (int l0 = 0; l0 < nl; l0 += LPR) {
         const int l = l0 + r;
         const bool act = t < B && l < nl;
@@ -68,6 +71,9 @@ __device__ __forceinline__ void line_fft(cplx<R> *buf, int nl, int ldist, int es
     }
 }
 
+// z columns per block of the column kernels: one round of line_fft
+template <int N> constexpr int col_tile() { return (FT / 64) * lines_per_wave<N>(); }
+
 template <typename R>
 __device__ __forceinline__ void load_twiddles(cplx<R> *dst, const double2 *src, int n) {
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -84,8 +90,7 @@ __global__ __launch_bounds__(FT) void k_col_bwd_fast(
     const double2 *__restrict__ psi, long ldp, cplx<R> *__restrict__ colbuf,
     const int *__restrict__ col_ptr, const int *__restrict__ ent_i, const int *__restrict__ ent_cz,
     const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb) {
-    constexpr int A = split_a(N3), B = N3 / A;
-    constexpr int CT = FT / B;
+    constexpr int CT = col_tile<N3>();
     constexpr int LS = N3 + 1;
     __shared__ cplx<R> tw[N3];
     __shared__ cplx<R> lines[CT * LS];
@@ -121,8 +126,7 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     const cplx<R> *__restrict__ colbuf, const int *__restrict__ col_ptr,
     const int *__restrict__ ent_i, const int *__restrict__ ent_cz, const double *__restrict__ ekpg,
     const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb, double inv_n, int acc) {
-    constexpr int A = split_a(N3), B = N3 / A;
-    constexpr int CT = FT / B;
+    constexpr int CT = col_tile<N3>();
     constexpr int LS = N3 + 1;
     __shared__ cplx<R> tw[N3];
     __shared__ cplx<R> lines[CT * LS];
@@ -131,6 +135,29 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     const int c0 = tile * CT;
     const int nc = min(CT, g.ncol - c0);
     load_twiddles(tw, tw3, N3);
+    const double2 *pb = psi + (long)band * ldp;
+    double2 *hb = hpsi + (long)band * ldh;
+    const int e0 = col_ptr[c0], e1 = col_ptr[c0 + nc];
+    // The first CF_PRE sphere entries of each thread: index, psi, ekpg and the
+    // non-local term in hpsi are loaded before the transform, so their two
+    // dependent global round trips overlap it.
+    constexpr int CF_PRE = 4;
+    int pcz[CF_PRE], pi[CF_PRE];
+    double2 pp[CF_PRE], ph[CF_PRE];
+    double pk[CF_PRE];
+#pragma unroll
+    for (int q = 0; q < CF_PRE; q++) {
+        const int e = e0 + (int)threadIdx.x + q * FT;
+        const int ec = e < e1 ? e : e0;  // clamped: always a valid entry
+        pcz[q] = ent_cz[ec];
+        pi[q] = ent_i[ec];
+    }
+#pragma unroll
+    for (int q = 0; q < CF_PRE; q++) {
+        pp[q] = pb[pi[q]];
+        pk[q] = ekpg[pi[q]];
+        ph[q] = acc ? hb[pi[q]] : make_double2(0.0, 0.0);
+    }
     const cplx<R> *ib = colbuf + (long)band * N3 * g.ncolp + c0;
     for (int t = threadIdx.x; t < N3 * CT; t += FT) {
         const int z = t / CT, cc = t - z * CT;
@@ -138,10 +165,18 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     }
     __syncthreads();
     line_fft<R, N3, -1>(lines, nc, LS, 1, tw);
-    const double2 *pb = psi + (long)band * ldp;
-    double2 *hb = hpsi + (long)band * ldh;
-    const int e0 = col_ptr[c0], e1 = col_ptr[c0 + nc];
-    for (int e = e0 + threadIdx.x; e < e1; e += FT) {
+    // hpsi = local/N + ekpg psi (+ the non-local term already in hpsi)
+#pragma unroll
+    for (int q = 0; q < CF_PRE; q++) {
+        if (e0 + (int)threadIdx.x + q * FT < e1) {
+            const cplx<R> v = lines[((pcz[q] >> 12) - c0) * LS + (pcz[q] & 0xfff)];
+            double2 h;
+            h.x = (double)v.x * inv_n + pk[q] * pp[q].x + ph[q].x;
+            h.y = (double)v.y * inv_n + pk[q] * pp[q].y + ph[q].y;
+            hb[pi[q]] = h;
+        }
+    }
+    for (int e = e0 + threadIdx.x + CF_PRE * FT; e < e1; e += FT) {
         const int cz = ent_cz[e];
         const int i = ent_i[e];
         const cplx<R> v = lines[((cz >> 12) - c0) * LS + (cz & 0xfff)];
@@ -150,7 +185,7 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
         double2 h;
         h.x = (double)v.x * inv_n + ek * p.x;
         h.y = (double)v.y * inv_n + ek * p.y;
-        if (acc) {  // the non-local term is already in hpsi
+        if (acc) {
             const double2 o = hb[i];
             h.x += o.x;
             h.y += o.y;
@@ -164,6 +199,12 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
 // V(r), forward x FFT, forward y FFT.  LDS: tw1[N1] tw2[N2] sub[N2][nxp]
 // rows[RB][N1+1] (sub/rows in dynamic shared memory, nxp depends on kd1).
 // ---------------------------------------------------------------------------
+// Row stride of the plane kernels' y/x-box sub-plane: nx box columns, then a
+// zero column and a trash column (nx is odd, so the stride is odd too).  No
+// further padding: at 72^3 four FP32 blocks (two FP64) fill the 160 KB of LDS
+// of a CU with a few hundred bytes to spare.
+__host__ __device__ __forceinline__ int plane_sub_stride(int nx) { return nx + 2; }
+
 // Row scratch of the x transforms: element p of a row at p + p/A1 (one pad per
 // A1 elements), so that the stride-A1 and stride-B1 accesses of the two passes
 // are odd strides in 16-byte units; rows RS apart.
@@ -183,7 +224,9 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
     __shared__ cplx<R> tw2[N2];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nx = g.nx, kd1 = g.kd[0];
-    const int nxp = nx | 1;
+    // sub rows: nx box columns, a column that stays zero (loads of x in the
+    // gap of the box) and a trash column (stores of x in the gap)
+    const int nxp = plane_sub_stride(nx);
     cplx<R> *sub = reinterpret_cast<cplx<R> *>(smem_raw);  // [N2][nxp]
     cplx<R> *rows = sub + N2 * nxp;                        // [RB][RS]
     const int z = blockIdx.x % g.n[2];
@@ -206,19 +249,25 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
     const int ushift = N1 - nx;
     const int r = threadIdx.x / B1, t = threadIdx.x - (threadIdx.x / B1) * B1;
     cplx<R> *row = rows + r * RS;
+    // sub column of x = t + A1 j for the loads (gap -> zero column) and the
+    // stores (gap -> trash column): fixed per thread for every row
+    int offl[B1], offs[B1];
+#pragma unroll
+    for (int j = 0; j < B1; j++) {
+        const int x = t + A1 * j;
+        const bool lo = x <= kd1, hi = x >= xhi;
+        offl[j] = lo ? x : hi ? x - ushift : nx;
+        offs[j] = lo ? x : hi ? x - ushift : nx + 1;
+    }
     for (int y0 = 0; y0 < N2; y0 += RB) {
         const int y = y0 + r;
         const bool act = r < RB && y < N2;
         cplx<R> v[B1 > A1 ? B1 : A1];
         // inverse x transform, pass 1: x = t + A1 j, only box x are nonzero
+        const cplx<R> *srow = sub + y * nxp;
         if (act && t < A1) {
 #pragma unroll
-            for (int j = 0; j < B1; j++) {
-                const int x = t + A1 * j;
-                v[j] = mk<R>(0, 0);
-                if (x <= kd1) v[j] = sub[y * nxp + x];
-                else if (x >= xhi) v[j] = sub[y * nxp + x - ushift];
-            }
+            for (int j = 0; j < B1; j++) v[j] = srow[offl[j]];
             dft<B1, +1, R>(v);
 #pragma unroll
             for (int k = 0; k < B1; k++) row[rpos<N1>(t + A1 * k)] = k == 0 ? v[0] : cmul(v[k], tw1[t * k]);
@@ -251,12 +300,9 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
 #pragma unroll
             for (int k = 0; k < B1; k++) v[k] = row[rpos<N1>(k + B1 * t)];
             dft<B1, -1, R>(v);
+            cplx<R> *wrow = sub + y * nxp;
 #pragma unroll
-            for (int p = 0; p < B1; p++) {
-                const int x = t + A1 * p;
-                if (x <= kd1) sub[y * nxp + x] = v[p];
-                else if (x >= xhi) sub[y * nxp + x - ushift] = v[p];
-            }
+            for (int p = 0; p < B1; p++) wrow[offs[p]] = v[p];
         }
         __syncthreads();
     }

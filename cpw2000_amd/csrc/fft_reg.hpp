@@ -24,15 +24,16 @@ namespace cpwgpu {
 
 template <int H> constexpr int dpp_ctrl() { return H == 1 ? 0xB1 : H == 2 ? 0x4E : 0x141; }
 
+// Every partner lane is in range and active, so the moves need no "old" value
+// (bound_ctrl set, all rows and banks enabled): no extra register copy.
 template <int CTRL> __device__ __forceinline__ float xlane(float x) {
     return __builtin_bit_cast(
-        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL> __device__ __forceinline__ double xlane(double x) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-    const unsigned hi =
-        (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
@@ -130,8 +131,8 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 // ---------------------------------------------------------------------------
 // plane stage, register variant: a block takes P planes (bands b0..b0+P-1 at
 // one z) with T threads and a register budget for W waves per SIMD.  LDS:
-// sub[P][N][S] (y rows of the x-box, stride S >= nx) and the twiddle table
-// twN[N].  Phases (barrier between each):
+// sub[P][N][S] (y rows of the x-box: nx box columns, a zero column and a trash
+// column, S = nx + 2) and the twiddle table twN[N].  Phases (barrier between each):
 //   gather columns -> inverse y (GY-lane lines over u) -> per row y: inverse
 //   x, V(r), forward x (GX-lane lines, no barrier) -> forward y -> scatter.
 // ---------------------------------------------------------------------------
@@ -189,23 +190,22 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
             const int p = task / N, y = task - p * N;
             cplx<R> *row = sub + (p * N + y) * S;
             cplx<R> v[MX];
+            // x = g + GX j: box columns, or the zero column nx for the gap
 #pragma unroll
             for (int j = 0; j < MX; j++) {
                 const int x = L.g + GX * j;
-                v[j] = mk<R>(0, 0);
-                if (x <= kd1) v[j] = row[x];
-                else if (x >= xhi) v[j] = row[x - ushift];
+                v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
             }
             line_dif<+1, GX, MX>(v, L, twN);
             const R *vy = vr + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
 #pragma unroll
             for (int k = 0; k < MX; k++) v[k] = scale(v[k], vy[k]);
             line_dit<-1, GX, MX>(v, L, twN);
+            // gap columns go to the trash column nx + 1
 #pragma unroll
             for (int m = 0; m < MX; m++) {
                 const int x = L.g + GX * m;
-                if (x <= kd1) row[x] = v[m];
-                else if (x >= xhi) row[x - ushift] = v[m];
+                row[x <= kd1 ? x : x >= xhi ? x - ushift : nx + 1] = v[m];
             }
         }
     }

@@ -530,6 +530,105 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_back_direct: the product of k_back without LDS.  Each wave owns 16 plane
+// waves and 64 bands (D^T = dhd^T anlga^T as in k_back, so an MFMA output row
+// is 16 consecutive plane waves); it loads its dhd fragments (small, L2
+// resident) and its projector values straight into registers, one chunk of
+// 4*BKPL projectors ahead of the MFMAs.  No barrier.
+// ---------------------------------------------------------------------------
+template <typename R> struct BackDirectCfg { static constexpr int KPL = 4; };
+template <> struct BackDirectCfg<double> { static constexpr int KPL = 2; };  // registers
+
+template <typename R>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void k_back_direct(const cplx<R> *__restrict__ anl,
+                                                                 long lda,
+                                                                 const cplx<R> *__restrict__ dhd,
+                                                                 double2 *__restrict__ hpsi, long ldh,
+                                                                 int mtxd, int nanl, int neig,
+                                                                 int beta) {
+    typedef typename MF<R>::v4 v4;
+    constexpr int KPL = BackDirectCfg<R>::KPL, KC = 4 * KPL;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int i = blockIdx.x * BP_BI + wave * 16 + lr;
+    const int n0 = blockIdx.y * BP_BN;
+    const int ic = min(i, mtxd - 1);
+    const bool iok = i < mtxd;
+    // dhd rows (bands) of this lane for the 4 row tiles, clamped
+    const cplx<R> *drow[4];
+    bool dok[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int n = n0 + t * 16 + lr;
+        dok[t] = n < neig;
+        drow[t] = dhd + (long)min(n, neig - 1) * nanl;
+    }
+    v4 P1[4], P2[4], P3[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        P1[t] = v4{0, 0, 0, 0};
+        P2[t] = v4{0, 0, 0, 0};
+        P3[t] = v4{0, 0, 0, 0};
+    }
+    cplx<R> d0[4][KPL], d1[4][KPL], a0[KPL], a1[KPL];
+    auto load = [&](cplx<R>(&d)[4][KPL], cplx<R>(&av)[KPL], int ka) {
+#pragma unroll
+        for (int u = 0; u < KPL; u++) {
+            const int a = ka + lg * KPL + u;
+            const int acl = min(a, nanl - 1);
+            const bool aok = a < nanl;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const cplx<R> v = drow[t][acl];
+                const bool ok = aok && dok[t];
+                d[t][u] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+            }
+            const cplx<R> w = anl[(long)acl * lda + ic];
+            const bool ok = aok && iok;
+            av[u] = mk<R>(ok ? w.x : (R)0, ok ? w.y : (R)0);
+        }
+    };
+    auto compute = [&](const cplx<R>(&d)[4][KPL], const cplx<R>(&av)[KPL]) {
+#pragma unroll
+        for (int u = 0; u < KPL; u++) {
+            const R cr = av[u].x, ci = av[u].y, cs = av[u].x + av[u].y;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                P1[t] = MF<R>::mma(d[t][u].x, cr, P1[t]);
+                P2[t] = MF<R>::mma(d[t][u].y, ci, P2[t]);
+                P3[t] = MF<R>::mma(d[t][u].x + d[t][u].y, cs, P3[t]);
+            }
+        }
+    };
+    load(d0, a0, 0);
+    for (int ka = 0; ka < nanl; ka += 2 * KC) {
+        const bool more1 = ka + KC < nanl, more2 = ka + 2 * KC < nanl;
+        if (more1) load(d1, a1, ka + KC);
+        compute(d0, a0);
+        if (more2) load(d0, a0, ka + 2 * KC);
+        if (more1) compute(d1, a1);
+    }
+    if (iok) {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int n = n0 + t * 16 + MF<R>::row(lg, r);
+                if (n < neig) {
+                    const R p1 = P1[t][r], p2 = P2[t][r], p3 = P3[t][r];
+                    double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
+                    if (beta) {
+                        const double2 o = hpsi[(long)n * ldh + i];
+                        h.x += o.x;
+                        h.y += o.y;
+                    }
+                    hpsi[(long)n * ldh + i] = h;
+                }
+            }
+    }
+}
+
 // FP64 -> FP32 copy of the projectors (FP32-mixed mode)
 __global__ void k_to_c64(const double2 *__restrict__ src, long lds, cplx<float> *__restrict__ dst,
                          long ldd, int rows, int cols) {

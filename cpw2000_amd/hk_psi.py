@@ -51,7 +51,7 @@ EXPORTS = (
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
-TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ = 0, 1, 2
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK = 0, 1, 2, 3
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
@@ -246,8 +246,9 @@ class Plan:
 
     def set_variant(self, prec, what, value):
         """Kernel choice for a precision: what = TUNE_PLANE (plane-stage kernel,
-        0 LDS two-pass, 1/2 register shapes, -1 default), TUNE_PROJ
-        (projection GEMM, 0 LDS-staged, 1 direct loads) or TUNE_SPLITK
+        0 LDS two-pass, 1..3 register shapes, -1 default), TUNE_PROJ
+        (projection GEMM, 0 LDS-staged, 1 direct loads), TUNE_BACK
+        (back-projection GEMM, same choices) or TUNE_SPLITK
         (projection GEMM split-K target blocks, -1 default)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 
@@ -269,6 +270,7 @@ class Plan:
                                       C.c_void_p(stream)))
         return {"plane": self.get_variant(prec, TUNE_PLANE),
                 "proj": self.get_variant(prec, TUNE_PROJ),
+                "back": self.get_variant(prec, TUNE_BACK),
                 "splitk": self.get_variant(prec, TUNE_SPLITK)}
 
     def set_potential(self, vscr):

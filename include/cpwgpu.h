@@ -108,21 +108,25 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 
 /* Tunable choices of a plan, per precision (CPWGPU_PREC_*):
  *   CPWGPU_TUNE_PLANE   kernel of the y/x plane stage: 0 = LDS two-pass lines,
- *                       1, 2 = register-resident lines (DPP butterflies) in two
- *                       shapes; -1 = the default for the mesh
+ *                       1..3 = register-resident lines (DPP butterflies) in
+ *                       three shapes; -1 = the default for the mesh
  *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
  *                       GEMM (64..4096); -1 = default (256)
  *   CPWGPU_TUNE_PROJ    kernel of the projection GEMM: 0 = LDS-staged tiles,
+ *                       1 = direct register loads; -1 = default (0)
+ *   CPWGPU_TUNE_BACK    kernel of the back-projection GEMM: 0 = LDS-staged D,
  *                       1 = direct register loads; -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2
+#define CPWGPU_TUNE_BACK 3
 int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
 /* Current choice (the value used by the next apply, defaults resolved). */
 int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);
 /* Autotuner: times cpwgpu_plan_apply for every plane variant built for the
- * mesh, then both projection kernels with a few split-K targets, on the
- * caller's arrays (neig bands, stream), and keeps the fastest in the plan.  hpsi_dev is overwritten with H psi.
+ * mesh, then both projection kernels with a few split-K targets, then both
+ * back-projection kernels, on the caller's arrays (neig bands, stream), and
+ * keeps the fastest in the plan.  hpsi_dev is overwritten with H psi.
  * If the environment variable CPWGPU_TUNE_CACHE names a file, choices are
  * read from it for meshes already tuned (no timing) and appended after
  * tuning; the key is (precision, n1, n2, n3, nx, ncol, nanl, neig). */

@@ -42,7 +42,7 @@ def hk():
 
 def _variants(hk, plan, prec):
     out = []
-    for v in range(3):
+    for v in range(4):
         try:
             plan.set_variant(prec, hk.TUNE_PLANE, v)
         except hk.CpwGpuError as e:
@@ -77,6 +77,7 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
             assert plan.get_variant(prec, hk.TUNE_PLANE) == v
             for jv in (0, 1):
                 plan.set_variant(prec, hk.TUNE_PROJ, jv)
+                plan.set_variant(prec, hk.TUNE_BACK, jv)
                 h = _apply(torch, plan, pt, prec).cpu().numpy().T
                 assert rel(h, href) < tol, (name, prec, v, jv, rel(h, href))
     plan.close()
@@ -95,6 +96,11 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             plan.set_variant(prec, hk.TUNE_PLANE, v)
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, v)
+        for bv in (0, 1):
+            plan.set_variant(prec, hk.TUNE_BACK, bv)
+            h = _apply(torch, plan, pt, prec)
+            assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "back", bv)
+        plan.set_variant(prec, hk.TUNE_BACK, -1)
         for jv in (0, 1):
             plan.set_variant(prec, hk.TUNE_PROJ, jv)
             for sk in (64, 128, 512, 1024, 4096):
@@ -110,8 +116,8 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
 def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
-    for what, value in ((hk.TUNE_PLANE, 3), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 2), (7, 0)):
+    for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
+                        (hk.TUNE_PROJ, 2), (hk.TUNE_BACK, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -133,10 +139,11 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         choice = plan.tune(pt, ht, prec)
         torch.cuda.synchronize()
         assert choice["plane"] in _variants(hk, plan, prec)
-        assert choice["proj"] in (0, 1)
+        assert choice["proj"] in (0, 1) and choice["back"] in (0, 1)
         assert choice["splitk"] in (128, 256, 512, 1024)
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
         plan.set_variant(prec, hk.TUNE_PROJ, choice["proj"])
+        plan.set_variant(prec, hk.TUNE_BACK, choice["back"])
         plan.set_variant(prec, hk.TUNE_SPLITK, choice["splitk"])
         # the tuner leaves H psi of its choice in hpsi
         assert float(torch.linalg.norm(ht - ref) / torch.linalg.norm(ref)) < tol
@@ -148,6 +155,7 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
     c2 = plan2.tune(pt, ht, hk.PREC_FP32)
     assert c2 == {"plane": plan.get_variant(hk.PREC_FP32, hk.TUNE_PLANE),
                   "proj": plan.get_variant(hk.PREC_FP32, hk.TUNE_PROJ),
+                  "back": plan.get_variant(hk.PREC_FP32, hk.TUNE_BACK),
                   "splitk": plan.get_variant(hk.PREC_FP32, hk.TUNE_SPLITK)}
     assert len([l for l in cache.read_text().split("\n") if l.strip()]) == 2
     plan.close()
