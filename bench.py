@@ -146,6 +146,102 @@ def cpu_baseline(d, seconds, nb=8):
     return out
 
 
+def bench_kpoints(torch, hk, synth, np, steps):
+    """16 k-points of the 64-atom cell on a shifted 2x2x4 grid (uneven
+    plane-wave counts), 128 bands each: the eager k loop against one launch of
+    the HIP graph that holds all of them.  Kernel choices come from tuning the
+    first k-point (same mesh and sizes for all)."""
+    spec = synth.config_spec(3)
+    nk = len(spec.kpts)
+    plans, psis, outs, mt = [], [], [], []
+    for k in range(nk):
+        d = synth.make_config(3, kindex=k)
+        p = hk.Plan.from_inputs(d)
+        x = torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda")
+        plans.append(p)
+        psis.append(x)
+        outs.append(torch.zeros_like(x))
+        mt.append(int(d["mtxd"]))
+    nb = psis[0].shape[0]
+    out = {"k_points": nk, "bands_per_k": nb, "plane_waves": [min(mt), max(mt)]}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        c = plans[0].tune(psis[0], outs[0], pid)
+        for p in plans[1:]:
+            for what, key in ((hk.TUNE_PLANE, "plane"), (hk.TUNE_PROJ, "proj"),
+                              (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk")):
+                p.set_variant(pid, what, c[key])
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        for _ in range(2):
+            for p, x, h in zip(plans, psis, outs):
+                p.apply(x, h, pid)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            for p, x, h in zip(plans, psis, outs):
+                p.apply(x, h, pid)
+        e1.record()
+        torch.cuda.synchronize()
+        eager = e0.elapsed_time(e1) / steps
+        g = hk.Graph(plans, psis, outs, pid)
+        for _ in range(2):
+            g.launch()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            g.launch()
+        e1.record()
+        torch.cuda.synchronize()
+        graph = e0.elapsed_time(e1) / steps
+        g.close()
+        out[prec] = {"eager_ms": round(eager, 4), "graph_ms": round(graph, 4),
+                     "eager_bands_per_s": round(nk * nb / (eager * 1e-3), 1),
+                     "graph_bands_per_s": round(nk * nb / (graph * 1e-3), 1), "tuned": c}
+    for p in plans:
+        p.close()
+    return out
+
+
+def bench_cfg2(torch, hk, synth, np, steps):
+    """216-atom Si (108^3 mesh, 864 projectors, 512 bands): throughput of both
+    paths and the FP32-mixed error against FP64 (H psi norm and Ritz values of
+    the band subspace; subspace matrices in FP64 on the GPU, eigenvalues of the
+    512 x 512 problem by LAPACK on the host)."""
+    import scipy.linalg as sla
+
+    d = synth.make_config(2)
+    plan = hk.Plan.from_inputs(d)
+    x = torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda")
+    nb = x.shape[0]
+    h = {}
+    out = {"mtxd": int(d["mtxd"]), "bands": nb, "nanl": int(d["nanl"]), "mesh": 108}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        h[prec] = torch.zeros_like(x)
+        c = plan.tune(x, h[prec], pid)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(steps):
+            plan.apply(x, h[prec], pid)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out[prec] = {"ms_per_step": round(ms, 4), "bands_per_s": round(nb / (ms * 1e-3), 1),
+                     "tuned": c}
+    out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
+    out["hpsi_rel_err_fp32"] = float(torch.linalg.norm(h["fp32"] - h["fp64"]) /
+                                     torch.linalg.norm(h["fp64"]))
+    s_mat = (x.conj() @ x.T).cpu().numpy()
+    eig = {}
+    for prec in ("fp64", "fp32"):
+        m = (x.conj() @ h[prec].T).cpu().numpy()
+        eig[prec] = sla.eigh(0.5 * (m + m.conj().T), s_mat, eigvals_only=True)
+    out["eig_err_ha"] = float(np.abs(eig["fp32"] - eig["fp64"]).max())
+    plan.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +249,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kpoints", action="store_true", help="skip the 16 k-point graph section")
+    ap.add_argument("--extra", action="store_true",
+                    help="also measure the 216-atom cell (108^3 mesh, 512 bands)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -255,6 +354,12 @@ def main():
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},
     }
+    if not args.no_kpoints:
+        log("bench: 16 k-points, eager loop vs one graph launch")
+        out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))
+    if args.extra:
+        log("bench: 216-atom cell (108^3, 512 bands)")
+        out["si216"] = bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))
     # the Fortran-facing drop-in: host arrays in and out (PCIe included)
     hk.set_precision(hk.PREC_FP64)
     hpsi_h = np.zeros_like(d["psi"], order="F")

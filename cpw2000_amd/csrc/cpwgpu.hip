@@ -623,46 +623,104 @@ int ensure_aux() {
     return CPWGPU_OK;
 }
 
+// Local and kinetic terms of one k-point on stream s, in band chunks that keep
+// colbuf below 4 GiB.  acc: the gather adds to the non-local term already in
+// hpsi; join (may be null): event the first gather waits for (that term).
 template <typename R>
-int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-            hipStream_t s) {
+int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+               bool acc, hipEvent_t join, hipStream_t s) {
     const FftGeom &g = p->g;
     const size_t cs = sizeof(cplx<R>);
     const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
     if (!vr) return fail(CPWGPU_ERR_ARG, "potential not set");
-    // bands per chunk so that colbuf stays below 4 GiB
     const size_t per_band = cs * (size_t)g.n[2] * g.ncolp;
     const int nbc = (int)std::max<size_t>(1, std::min<size_t>(neig, (4ull << 30) / per_band));
     int st = p->colbuf.reserve(per_band * nbc);
     if (st) return st;
     cplx<R> *colbuf = (cplx<R> *)p->colbuf.p;
-    const bool nl = p->nanl > 0 && g.mtxd > 0;
-    if (nl && !g_overlap) {
-        // serial: non-local term first (writes hpsi), then the local term adds
-        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, s))) return st;
-    } else if (nl) {
-        // fork: non-local term on the second stream, written into hpsi
-        if ((st = ensure_aux())) return st;
-        HIPCHK(hipEventRecord(g_fork, s));
-        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
-        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, g_aux))) return st;
-        HIPCHK(hipEventRecord(g_join, g_aux));
-    }
-    bool joined = !nl || !g_overlap;
+    bool joined = join == nullptr;
     for (int b0 = 0; b0 < neig; b0 += nbc) {
         const int nb = std::min(nbc, neig - b0);
         const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
-                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, nl ? 1 : 0};
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, acc ? 1 : 0};
         if (g.ncol > 0) {
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, s))) return st;
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, s))) return st;
         }
         if (!joined) {
             // join: the gather adds the local and kinetic terms to the non-local one
-            HIPCHK(hipStreamWaitEvent(s, g_join, 0));
+            HIPCHK(hipStreamWaitEvent(s, join, 0));
             joined = true;
         }
         if (g.ncol > 0 && (st = local_stage<R>(p, la, colbuf, vr, ST_COL_FWD, s))) return st;
+    }
+    return CPWGPU_OK;
+}
+
+template <typename R>
+int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+            hipStream_t s) {
+    const bool nl = p->nanl > 0 && p->g.mtxd > 0;
+    int st;
+    if (nl && !g_overlap) {
+        // serial: non-local term first (writes hpsi), then the local term adds
+        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, s))) return st;
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s);
+    }
+    if (nl) {
+        // fork: non-local term on the second stream, written into hpsi
+        if ((st = ensure_aux())) return st;
+        HIPCHK(hipEventRecord(g_fork, s));
+        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, g_aux))) return st;
+        HIPCHK(hipEventRecord(g_join, g_aux));
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s);
+    }
+    return local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s);
+}
+
+// join events of the batched apply, one per k-point
+std::vector<hipEvent_t> g_joins;
+
+// Several k-points: the non-local terms of all k-points run one after another
+// on the second stream while the local terms run on s; the gather of k-point k
+// waits for the non-local term of k only, so the two streams stay busy across
+// k-point boundaries.  Replaces the k loop of Src/cpw/cpw_scf.f90:488-544
+// around hk_psi_c16 for a batch of k-points.
+template <typename R>
+int apply_batch_t(int nk, cpwgpu_plan_s *const *plans, const int *neig, const double *const *psi,
+                  const long *ldp, double *const *hpsi, const long *ldh, hipStream_t s) {
+    int st;
+    if (!g_overlap) {
+        for (int k = 0; k < nk; k++)
+            if (neig[k] > 0 && (st = apply_t<R>(plans[k], neig[k], psi[k], ldp[k], hpsi[k], ldh[k], s)))
+                return st;
+        return CPWGPU_OK;
+    }
+    if ((st = ensure_aux())) return st;
+    while ((int)g_joins.size() < nk) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        g_joins.push_back(e);
+    }
+    bool forked = false;
+    for (int k = 0; k < nk; k++) {
+        cpwgpu_plan_s *p = plans[k];
+        if (neig[k] == 0 || p->nanl == 0 || p->g.mtxd == 0) continue;
+        if (!forked) {
+            HIPCHK(hipEventRecord(g_fork, s));
+            HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+            forked = true;
+        }
+        if ((st = nonlocal<R>(p, neig[k], psi[k], ldp[k], hpsi[k], ldh[k], 0, g_aux))) return st;
+        HIPCHK(hipEventRecord(g_joins[k], g_aux));
+    }
+    for (int k = 0; k < nk; k++) {
+        cpwgpu_plan_s *p = plans[k];
+        if (neig[k] == 0) continue;
+        const bool nl = p->nanl > 0 && p->g.mtxd > 0;
+        if ((st = local_part<R>(p, neig[k], psi[k], ldp[k], hpsi[k], ldh[k], nl, nl ? g_joins[k] : nullptr, s)))
+            return st;
     }
     return CPWGPU_OK;
 }
@@ -1291,23 +1349,21 @@ int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plan
     for (int k = 0; k < nk; k++)
         if (!plans[k] || neig[k] < 0 || ldp[k] < plans[k]->mtxd || ldh[k] < plans[k]->mtxd)
             return fail(CPWGPU_ERR_ARG, "bad arguments for k-point %d", k);
-    auto apply = [&](int k, hipStream_t s) -> int {
-        if (neig[k] == 0) return CPWGPU_OK;
+    auto apply = [&](hipStream_t s) -> int {
         return prec == CPWGPU_PREC_FP64
-                   ? apply_t<double>(plans[k], neig[k], psi[k], ldp[k], hpsi[k], ldh[k], s)
-                   : apply_t<float>(plans[k], neig[k], psi[k], ldp[k], hpsi[k], ldh[k], s);
+                   ? apply_batch_t<double>(nk, plans, neig, psi, ldp, hpsi, ldh, s)
+                   : apply_batch_t<float>(nk, plans, neig, psi, ldp, hpsi, ldh, s);
     };
     // eager pass: every buffer, attribute and FP32 projector copy exists before capture
     const bool prof = g_prof.on;
     g_prof.on = false;
-    for (int k = 0; k < nk; k++)
-        if ((st = apply(k, g_stream))) { g_prof.on = prof; return st; }
+    if ((st = apply(g_stream))) { g_prof.on = prof; return st; }
     HIPCHK(hipStreamSynchronize(g_stream));
     hipStream_t cs = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     cpwgpu_graph_s *gr = new cpwgpu_graph_s();
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    for (int k = 0; e == hipSuccess && k < nk && st == CPWGPU_OK; k++) st = apply(k, cs);
+    if (e == hipSuccess) st = apply(cs);
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &gr->graph) : e;
     g_prof.on = prof;
     if (e == hipSuccess && e2 == hipSuccess && st == CPWGPU_OK)

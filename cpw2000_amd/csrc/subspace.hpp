@@ -169,14 +169,20 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
                 if (q < n) {
                     const double app = a[p + (long)p * n].x, aqq = a[q + (long)q * n].x;
                     const double2 apq = a[p + (long)q * n];
-                    const double g = sqrt(apq.x * apq.x + apq.y * apq.y);
+                    // |apq| without squaring: once the element has decayed
+                    // below ~1e-154, apq.x^2 + apq.y^2 is denormal and keeps
+                    // only a few digits, |u| drifts from 1 and the phase
+                    // factor would rescale row and column q
+                    const double g = hypot(apq.x, apq.y);
                     if (g > 1e-300) {
-                        u = make_double2(apq.x / g, -apq.y / g);  // exp(-i phi)
                         const double zeta = (aqq - app) / (2.0 * g);
                         const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
                                          (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        sn = t * c;
+                        if (t != 0.0) {  // a zero angle is the identity: no phase either
+                            u = make_double2(apq.x / g, -apq.y / g);  // exp(-i phi)
+                            c = 1.0 / sqrt(1.0 + t * t);
+                            sn = t * c;
+                        }
                     }
                 }
                 rc[k] = c;

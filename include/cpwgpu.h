@@ -138,8 +138,10 @@ int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long l
 typedef struct cpwgpu_graph_s *cpwgpu_graph_t;
 /* Records H|psi> for nk k-points (plans[k], neig[k] bands psi[k](ldp[k]) ->
  * hpsi[k](ldh[k]) on device, each k with its own plane-wave count) into one
- * HIP graph.  Each plan is applied once eagerly first (buffers, kernel
- * attributes), then the applies are captured; the device arrays must stay
+ * HIP graph.  The non-local GEMMs of all k-points are chained on a second
+ * stream beside the FFT chains, each gather waiting only for its own
+ * k-point's non-local term.  The batch is applied once eagerly first
+ * (buffers, kernel attributes), then captured; the device arrays must stay
  * valid while the graph exists.  Replaces the k-loop of
  * Src/cpw/cpw_scf.f90:488-544 around hk_psi_c16 for a batch of k-points. */
 int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plans, const int *neig,

@@ -42,7 +42,7 @@ def _apply(torch, plan, psi_t, prec):
     return h
 
 
-@pytest.mark.parametrize("cfg,nb", [(0, 16), (0, 7), (0, 1), (1, 33)])
+@pytest.mark.parametrize("cfg,nb", [(0, 16), (0, 7), (0, 1), (1, 33), (1, 128)])
 def test_rayleigh_ritz_matches_lapack(torch_dev, hk, cfg, nb):
     torch = torch_dev
     d = synth.make_config(cfg, nbands=max(nb, 2))
@@ -136,3 +136,21 @@ def test_block_iteration_fp32_mixed_with_fp64_refinement(torch_dev, hk, orc):
     eig64, res64 = hk.rayleigh_ritz(psi, hpsi)
     assert np.abs(eig64[:8] - exact).max() < 1e-6
     plan.close()
+
+
+@pytest.mark.parametrize("nb", [64, 96, 128])
+def test_eigensolver_on_dense_hermitian_subspace(torch_dev, hk, nb):
+    """Orthonormal psi and H psi = psi A (A dense Hermitian): the Ritz values
+    are the eigenvalues of A.  The Jacobi sweeps run past convergence here
+    (off-diagonal elements decay into the denormal range); every rotation must
+    stay unitary (regression: |a_pq| computed by squaring lost its digits
+    there and the phase factor rescaled rows and columns)."""
+    torch = torch_dev
+    d = synth.make_config(1)
+    x = _dev(torch, d["psi"][: d["mtxd"], :nb])
+    rng = np.random.default_rng(nb)
+    a = rng.normal(size=(nb, nb)) + 1j * rng.normal(size=(nb, nb))
+    a = 0.5 * (a + a.conj().T)
+    hx = (torch.from_numpy(a).to("cuda").T @ x).contiguous()
+    eig, _ = hk.rayleigh_ritz(x.clone(), hx, rotate=False, residuals=False)
+    assert np.abs(eig - np.linalg.eigvalsh(a)).max() < 1e-11 * np.abs(a).sum() / nb
