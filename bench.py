@@ -252,6 +252,9 @@ def main():
     ap.add_argument("--no-kpoints", action="store_true", help="skip the 16 k-point graph section")
     ap.add_argument("--extra", action="store_true",
                     help="also measure the 216-atom cell (108^3 mesh, 512 bands)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed loops, one plain 512 MiB device copy (a known-size "
+                         "dispatch that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -354,6 +357,12 @@ def main():
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},
     }
+    if args.calib_copy:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)
+        dst = src.clone()  # 512 MiB read + 512 MiB written
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_kpoints:
         log("bench: 16 k-points, eager loop vs one graph launch")
         out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))

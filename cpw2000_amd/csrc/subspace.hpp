@@ -115,22 +115,35 @@ __global__ __launch_bounds__(SS_THREADS) void k_to_standard(const double2 *l, do
 // Cyclic two-sided Jacobi for the Hermitian matrix a (destroyed): on exit the
 // eigenvalues ascend in eig and v holds the eigenvectors (columns).  Each
 // round applies n/2 disjoint rotations at once (round-robin pairing); a sweep
-// is n-1 rounds; stops when off(a) <= tol*|a| or after max_sweeps sweeps.
-// rot is 3*n doubles of scratch (LDS would limit n).
+// is n-1 rounds; stops when off(a) <= tol*|a| or after max_sweeps sweeps;
+// elements below tol*|a|/n are not rotated.  Round r pairs the positions
+// (slot k, slot np-1-k) of the rotating schedule: slot 0 holds index 0 and
+// slot i >= 1 holds index 1 + (i - 1 - r) mod (np - 1).  pos is n ints of
+// scratch (the final ordering), rot is unused (kept for the workspace layout).
+__device__ __forceinline__ int rr_slot(int i, int r, int np) {
+    if (i == 0) return 0;
+    int m = (i - 1 - r) % (np - 1);
+    if (m < 0) m += np - 1;
+    return 1 + m;
+}
+
+constexpr int JAC_MAXPAIR = 1024;  // n <= 2048
+
 __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, double *eig, int n,
                                                        int max_sweeps, double tol, int *pos,
                                                        double *rot, int *sweeps_out) {
+    (void)rot;
     __shared__ double red[SS_THREADS];
     __shared__ int stop;
+    __shared__ double rc[JAC_MAXPAIR], rs[JAC_MAXPAIR];
+    __shared__ double2 ru[JAC_MAXPAIR];
+    __shared__ short rp[JAC_MAXPAIR], rq[JAC_MAXPAIR];
     const int np = (n + 1) & ~1;  // even number of positions (one dummy if n odd)
     for (long idx = threadIdx.x; idx < (long)n * n; idx += blockDim.x) {
         const int i = (int)(idx % n), j = (int)(idx / n);
         v[idx] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
-    for (int i = threadIdx.x; i < np; i += blockDim.x) pos[i] = i;
     __syncthreads();
-    double *rc = rot, *rs = rot + np / 2;
-    double2 *ru = reinterpret_cast<double2 *>(rot + np);
     int sweep = 0;
     for (; sweep < max_sweeps; sweep++) {
         // convergence test: off-diagonal vs total Frobenius norm
@@ -157,12 +170,16 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
             __syncthreads();
         }
         if (threadIdx.x == 0) stop = off_all <= tol * tol * red[0] || off_all == 0.0;
+        // pairs with |a_pq| below thr are left alone: together they stay below
+        // the stopping bound (n^2 thr^2 = tol^2 |a|^2), and rotating them only
+        // stirs rounding noise that keeps the sweeps from ever meeting it
+        const double thr = fmax(1e-300, tol * sqrt(red[0]) / n);
         __syncthreads();
         if (stop) break;
         for (int round = 0; round < np - 1; round++) {
             // rotation parameters of the pairs of this round
             for (int k = threadIdx.x; k < np / 2; k += blockDim.x) {
-                int p = pos[k], q = pos[np - 1 - k];
+                int p = rr_slot(k, round, np), q = rr_slot(np - 1 - k, round, np);
                 if (p > q) { const int t = p; p = q; q = t; }
                 double c = 1.0, sn = 0.0;
                 double2 u = make_double2(1.0, 0.0);
@@ -174,7 +191,7 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
                     // only a few digits, |u| drifts from 1 and the phase
                     // factor would rescale row and column q
                     const double g = hypot(apq.x, apq.y);
-                    if (g > 1e-300) {
+                    if (g > thr) {
                         const double zeta = (aqq - app) / (2.0 * g);
                         const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
                                          (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -188,15 +205,17 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
                 rc[k] = c;
                 rs[k] = sn;
                 ru[k] = u;
+                rp[k] = (short)p;
+                rq[k] = (short)q;
             }
             __syncthreads();
             // rows: p' = c p - conj(u) s q ; q' = s p + conj(u) c q
             for (long idx = threadIdx.x; idx < (long)(np / 2) * n; idx += blockDim.x) {
                 const int k = (int)(idx / n), j = (int)(idx % n);
-                int p = pos[k], q = pos[np - 1 - k];
-                if (p > q) { const int t = p; p = q; q = t; }
-                if (q >= n) continue;
-                const double c = rc[k], s = rs[k];
+                const int p = rp[k], q = rq[k];
+                const double s = rs[k];
+                if (q >= n || (s == 0.0 && ru[k].y == 0.0 && ru[k].x == 1.0)) continue;  // identity
+                const double c = rc[k];
                 const double2 uc = zconj(ru[k]);
                 const double2 ap = a[p + (long)j * n], aq = a[q + (long)j * n];
                 const double2 t = zmul(uc, aq);
@@ -207,10 +226,10 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
             // columns (of a and v): p' = c p - u s q ; q' = s p + u c q
             for (long idx = threadIdx.x; idx < (long)(np / 2) * n; idx += blockDim.x) {
                 const int k = (int)(idx / n), i = (int)(idx % n);
-                int p = pos[k], q = pos[np - 1 - k];
-                if (p > q) { const int t = p; p = q; q = t; }
-                if (q >= n) continue;
-                const double c = rc[k], s = rs[k];
+                const int p = rp[k], q = rq[k];
+                const double s = rs[k];
+                if (q >= n || (s == 0.0 && ru[k].y == 0.0 && ru[k].x == 1.0)) continue;  // identity
+                const double c = rc[k];
                 const double2 u = ru[k];
                 double2 ap = a[i + (long)p * n], aq = a[i + (long)q * n];
                 double2 t = zmul(u, aq);
@@ -221,13 +240,6 @@ __global__ __launch_bounds__(SS_THREADS) void k_jacobi(double2 *a, double2 *v, d
                 t = zmul(u, aq);
                 v[i + (long)p * n] = make_double2(c * ap.x - s * t.x, c * ap.y - s * t.y);
                 v[i + (long)q * n] = make_double2(s * ap.x + c * t.x, s * ap.y + c * t.y);
-            }
-            __syncthreads();
-            // next pairing: keep position 0, rotate 1..np-1
-            if (threadIdx.x == 0) {
-                const int last = pos[np - 1];
-                for (int i = np - 1; i > 1; i--) pos[i] = pos[i - 1];
-                pos[1] = last;
             }
             __syncthreads();
         }

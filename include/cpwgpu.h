@@ -138,11 +138,10 @@ int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long l
 typedef struct cpwgpu_graph_s *cpwgpu_graph_t;
 /* Records H|psi> for nk k-points (plans[k], neig[k] bands psi[k](ldp[k]) ->
  * hpsi[k](ldh[k]) on device, each k with its own plane-wave count) into one
- * HIP graph.  The non-local GEMMs of all k-points are chained on a second
- * stream beside the FFT chains, each gather waiting only for its own
- * k-point's non-local term.  The batch is applied once eagerly first
- * (buffers, kernel attributes), then captured; the device arrays must stay
- * valid while the graph exists.  Replaces the k-loop of
+ * HIP graph (each k-point's non-local GEMMs overlap its FFT chain, as in
+ * cpwgpu_plan_apply).  The batch is applied once eagerly first (buffers,
+ * kernel attributes), then captured; the device arrays must stay valid while
+ * the graph exists.  Replaces the k-loop of
  * Src/cpw/cpw_scf.f90:488-544 around hk_psi_c16 for a batch of k-points. */
 int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plans, const int *neig,
                         const double *const *psi, const long *ldp, double *const *hpsi,
@@ -161,7 +160,7 @@ int cpwgpu_graph_destroy(cpwgpu_graph_t graph);
  * res (nb, may be NULL: res_n = ||hpsi_n - eig_n psi_n||^2 after the
  * rotation) are host arrays.  Returns when done.  Replaces the FP64 steps of
  * Src/ditsp/ditsp_c16_ritz.f90:157-287 (orthonormalisation, H_sub, zheevd,
- * rotation) and the residual of Src/ditsp/ditsp_error.f90:70-76. */
+ * rotation) and the residual of Src/ditsp/ditsp_error.f90:70-76.  nb <= 2048. */
 int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hpsi_dev, long ldh,
                          int rotate, double *eig, double *res, void *stream);
 
