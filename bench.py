@@ -79,6 +79,23 @@ def roofline_entry(name, work, avg_ms, nb, prec):
             "avg_ms": round(avg_ms, 5), "algorithmic_flops": w["flops"] * nb}
 
 
+def measured_traffic(prec, plane_variant, n1):
+    """HBM bytes per launch of the plane-stage kernel measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on this
+    workload, recorded in profiles/roofline_traffic.json; None if that file has
+    no entry for this kernel."""
+    path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    e = rec.get("kernels", {}).get(f"{prec}/plane{plane_variant}/{n1}")
+    if not e:
+        return None, None
+    return e["bytes"], f"rocprofv3 FETCH_SIZE+WRITE_SIZE of {e['kernel']} (profiles/roofline_traffic.json)"
+
+
 def time_path(torch, hk, plan, psi, hpsi, prec, steps, warmup, kernels):
     """Times `steps` applies; returns ms/step and per-kernel average ms."""
     for _ in range(warmup):
@@ -369,6 +386,11 @@ def main():
     if args.extra:
         log("bench: 216-atom cell (108^3, 512 bands)")
         out["si216"] = bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))
+    for key, prec in (("roofline", "fp32"), ("roofline_fp64", "fp64")):
+        if out[key]["kernel"] == "k_plane":
+            tb, src = measured_traffic(prec, tuned[prec]["plane"], info["n1"])
+            out[key]["traffic"] = tb
+            out[key]["traffic_source"] = src
     # the Fortran-facing drop-in: host arrays in and out (PCIe included)
     hk.set_precision(hk.PREC_FP64)
     hpsi_h = np.zeros_like(d["psi"], order="F")
