@@ -565,11 +565,14 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
     }
     const long lda = sizeof(R) == 8 ? p->lda : p->lda32;
     const int ix = sizeof(R) == 4 ? 1 : 0;
-    const bool direct = p->proj_var[ix] == 1;
-    const int BM = direct ? direct_bm<R>() : proj_bm<R>(), BN = direct ? direct_bn<R>() : proj_bn<R>();
+    const int pv = p->proj_var[ix] >= 1 && p->proj_var[ix] <= NPROJ_DIRECT ? p->proj_var[ix] : 0;
+    // block tile and K granule of the chosen projection kernel
+    int BM = proj_bm<R>(), BN = proj_bn<R>(), BK = GemmCfg<R>::BK;
+#define CPW_PD(V) if (pv == V) { BM = direct_bm<R, V>(); BN = direct_bn<R, V>(); BK = direct_bk<R, V>(); }
+    CPW_PD(1) CPW_PD(2) CPW_PD(3) CPW_PD(4)
+#undef CPW_PD
     const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
     // K slices are whole LDS stages of k_proj / whole chunks of k_proj_direct
-    const int BK = direct ? 4 * DirectCfg<R>::KPL : GemmCfg<R>::BK;
     const int ksteps = (g.mtxd + BK - 1) / BK;
     const int target = p->splitk[ix] > 0 ? p->splitk[ix] : 256;
     int nsl = std::max(1, std::min(ksteps, (target + tm * tn - 1) / (tm * tn)));
@@ -579,10 +582,17 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
     if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
     {
         ProfScope ps_(CPWGPU_K_PROJ, s);
-        if (direct)
-            hipLaunchKernelGGL(k_proj_direct<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig,
-                               kslice);
+        const dim3 grid(tm, tn, nsl);
+        const double2 *ps = (const double2 *)psi;
+        cplx<R> *pt = (cplx<R> *)p->part.p;
+        if (pv == 1)
+            hipLaunchKernelGGL((k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+        else if (pv == 2)
+            hipLaunchKernelGGL((k_proj_direct<R, 2>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+        else if (pv == 3)
+            hipLaunchKernelGGL((k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+        else if (pv == 4)
+            hipLaunchKernelGGL((k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
         else
             hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
                                (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig,
@@ -595,15 +605,20 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
     }
     {
         ProfScope ps_(CPWGPU_K_BACK, s);
-        const dim3 grid((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN);
-        if (p->back_var[ix] == 1)
-            hipLaunchKernelGGL(k_back_direct<R>, grid, dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const cplx<R> *)p->dhd.p, (double2 *)hpsi, ldh, g.mtxd, nanl, neig,
-                               beta);
-        else
-            hipLaunchKernelGGL(k_back<R>, grid, dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const cplx<R> *)p->dhd.p, (double2 *)hpsi, ldh, g.mtxd, nanl, neig,
-                               beta);
+        const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_DIRECT ? p->back_var[ix] : 0;
+        const cplx<R> *dh = (const cplx<R> *)p->dhd.p;
+        double2 *hp = (double2 *)hpsi;
+#define CPW_BD(V)                                                                                  \
+    if (bv == V)                                                                                   \
+        hipLaunchKernelGGL((k_back_direct<R, V>),                                                  \
+                           dim3((g.mtxd + back_bi<R, V>() - 1) / back_bi<R, V>(),                  \
+                                (neig + back_bn<R, V>() - 1) / back_bn<R, V>()),                   \
+                           dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta);
+        CPW_BD(1) CPW_BD(2) CPW_BD(3)
+#undef CPW_BD
+        if (bv == 0)
+            hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta);
     }
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
@@ -1071,8 +1086,8 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
         p->plane_var[ix] = ok ? pv : -1;
-        p->proj_var[ix] = jv == 0 || jv == 1 ? jv : -1;
-        p->back_var[ix] = bv == 0 || bv == 1 ? bv : -1;
+        p->proj_var[ix] = jv >= 0 && jv <= NPROJ_DIRECT ? jv : -1;
+        p->back_var[ix] = bv >= 0 && bv <= NBACK_DIRECT ? bv : -1;
         p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
         return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
     }
@@ -1094,7 +1109,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         // projection kernel x split-K target
         const int cand[4] = {128, 256, 512, 1024};
         best = 3.4e38f;
-        for (int jv = 0; jv < 2; jv++)
+        for (int jv = 0; jv <= NPROJ_DIRECT; jv++)
             for (int c : cand) {
                 p->proj_var[ix] = jv;
                 p->splitk[ix] = c;
@@ -1107,7 +1122,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     int best_bv = 0;
     if (p->nanl > 0) {
         best = 3.4e38f;
-        for (int bv = 0; bv < 2; bv++) {
+        for (int bv = 0; bv <= NBACK_DIRECT; bv++) {
             p->back_var[ix] = bv;
             if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
             if (t < best) { best = t; best_bv = bv; }
@@ -1397,12 +1412,12 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_PROJ) {
-        if (value > 1) return fail(CPWGPU_ERR_MODE, "projection kernel %d", value);
+        if (value > NPROJ_DIRECT) return fail(CPWGPU_ERR_MODE, "projection kernel %d", value);
         plan->proj_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_BACK) {
-        if (value > 1) return fail(CPWGPU_ERR_MODE, "back-projection kernel %d", value);
+        if (value > NBACK_DIRECT) return fail(CPWGPU_ERR_MODE, "back-projection kernel %d", value);
         plan->back_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
@@ -1430,11 +1445,11 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_PROJ) {
-        *value = plan->proj_var[ix] == 1 ? 1 : 0;
+        *value = plan->proj_var[ix] >= 1 && plan->proj_var[ix] <= NPROJ_DIRECT ? plan->proj_var[ix] : 0;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_BACK) {
-        *value = plan->back_var[ix] == 1 ? 1 : 0;
+        *value = plan->back_var[ix] >= 1 && plan->back_var[ix] <= NBACK_DIRECT ? plan->back_var[ix] : 0;
         return CPWGPU_OK;
     }
     return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);

@@ -208,13 +208,23 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
 // even for complex64).  Full chunks use 16-byte vector loads; the slice tail
 // falls back to clamped, zero-masked element loads.
 // ---------------------------------------------------------------------------
-// wave tile (WM*16) x (WN*16), KPL K values per lane per chunk; sized so that
-// the accumulators and two chunks of loads stay within 256 registers
-template <typename R> struct DirectCfg;
-template <> struct DirectCfg<float> { static constexpr int WM = 4, WN = 2, KPL = 4; };
-template <> struct DirectCfg<double> { static constexpr int WM = 2, WN = 2, KPL = 4; };
-template <typename R> constexpr int direct_bm() { return 2 * DirectCfg<R>::WM * 16; }
-template <typename R> constexpr int direct_bn() { return 2 * DirectCfg<R>::WN * 16; }
+// Shapes of k_proj_direct (autotuner choices 1..NPROJ_DIRECT): wave tile
+// (WM*16) x (WN*16), KPL K values per lane per chunk, register budget for W
+// waves per SIMD (accumulators plus two chunks of loads).
+constexpr int NPROJ_DIRECT = 4;
+template <typename R, int V> struct DirectCfg;
+// (all compile without register spills on gfx950)
+template <> struct DirectCfg<float, 1> { static constexpr int WM = 4, WN = 2, KPL = 2, W = 2; };
+template <> struct DirectCfg<float, 2> { static constexpr int WM = 4, WN = 2, KPL = 4, W = 1; };
+template <> struct DirectCfg<float, 3> { static constexpr int WM = 2, WN = 2, KPL = 8, W = 1; };
+template <> struct DirectCfg<float, 4> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 2; };
+template <> struct DirectCfg<double, 1> { static constexpr int WM = 2, WN = 2, KPL = 2, W = 2; };
+template <> struct DirectCfg<double, 2> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 1; };
+template <> struct DirectCfg<double, 3> { static constexpr int WM = 4, WN = 1, KPL = 2, W = 2; };
+template <> struct DirectCfg<double, 4> { static constexpr int WM = 2, WN = 1, KPL = 4, W = 2; };
+template <typename R, int V> constexpr int direct_bm() { return 2 * DirectCfg<R, V>::WM * 16; }
+template <typename R, int V> constexpr int direct_bn() { return 2 * DirectCfg<R, V>::WN * 16; }
+template <typename R, int V> constexpr int direct_bk() { return 4 * DirectCfg<R, V>::KPL; }
 
 template <typename R, int KPL>
 __device__ __forceinline__ void load_row_chunk(cplx<R> (&dst)[KPL], const cplx<R> *__restrict__ row,
@@ -259,17 +269,14 @@ __device__ __forceinline__ void load_psi_chunk(double2 (&dst)[KPL], const double
     }
 }
 
-template <typename R>
-__global__ __launch_bounds__(GEMM_THREADS, 2) void k_proj_direct(const cplx<R> *__restrict__ anl,
-                                                              long lda,
-                                                              const double2 *__restrict__ psi,
-                                                              long ldp, cplx<R> *__restrict__ part,
-                                                              int mtxd, int nanl, int neig,
-                                                              int kslice) {
+template <typename R, int V>
+__global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_direct(
+    const cplx<R> *__restrict__ anl, long lda, const double2 *__restrict__ psi, long ldp,
+    cplx<R> *__restrict__ part, int mtxd, int nanl, int neig, int kslice) {
     typedef typename MF<R>::v4 v4;
-    constexpr int WM = DirectCfg<R>::WM, WN = DirectCfg<R>::WN;
-    constexpr int KPL = DirectCfg<R>::KPL, KC = 4 * KPL;
-    constexpr int BM = direct_bm<R>(), BN = direct_bn<R>();
+    constexpr int WM = DirectCfg<R, V>::WM, WN = DirectCfg<R, V>::WN;
+    constexpr int KPL = DirectCfg<R, V>::KPL, KC = 4 * KPL;
+    constexpr int BM = direct_bm<R, V>(), BN = direct_bn<R, V>();
     const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -537,67 +544,99 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
 // resident) and its projector values straight into registers, one chunk of
 // 4*BKPL projectors ahead of the MFMAs.  No barrier.
 // ---------------------------------------------------------------------------
-template <typename R> struct BackDirectCfg { static constexpr int KPL = 4; };
-template <> struct BackDirectCfg<double> { static constexpr int KPL = 2; };  // registers
+// Shapes of k_back_direct (autotuner choices 1..NBACK_DIRECT): a wave owns NT
+// tiles of 16 bands and IT tiles of 16 plane waves, loads KPL projectors per
+// lane per chunk, register budget for W waves per SIMD.  A block is 4 waves
+// along the plane waves: 64*IT plane waves x 16*NT bands.
+constexpr int NBACK_DIRECT = 3;
+template <typename R, int V> struct BackDirectCfg;
+// (all compile without register spills on gfx950)
+template <> struct BackDirectCfg<float, 1> { static constexpr int NT = 4, IT = 1, KPL = 4, W = 2; };
+template <> struct BackDirectCfg<float, 2> { static constexpr int NT = 4, IT = 2, KPL = 4, W = 2; };
+template <> struct BackDirectCfg<float, 3> { static constexpr int NT = 2, IT = 2, KPL = 4, W = 3; };
+template <> struct BackDirectCfg<double, 1> { static constexpr int NT = 4, IT = 1, KPL = 2, W = 2; };
+template <> struct BackDirectCfg<double, 2> { static constexpr int NT = 2, IT = 2, KPL = 2, W = 2; };
+template <> struct BackDirectCfg<double, 3> { static constexpr int NT = 2, IT = 1, KPL = 2, W = 3; };
+template <typename R, int V> constexpr int back_bi() { return 4 * 16 * BackDirectCfg<R, V>::IT; }
+template <typename R, int V> constexpr int back_bn() { return 16 * BackDirectCfg<R, V>::NT; }
 
-template <typename R>
-__global__ __launch_bounds__(GEMM_THREADS, 2) void k_back_direct(const cplx<R> *__restrict__ anl,
-                                                                 long lda,
-                                                                 const cplx<R> *__restrict__ dhd,
-                                                                 double2 *__restrict__ hpsi, long ldh,
-                                                                 int mtxd, int nanl, int neig,
-                                                                 int beta) {
+template <typename R, int V>
+__global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back_direct(
+    const cplx<R> *__restrict__ anl, long lda, const cplx<R> *__restrict__ dhd,
+    double2 *__restrict__ hpsi, long ldh, int mtxd, int nanl, int neig, int beta) {
     typedef typename MF<R>::v4 v4;
-    constexpr int KPL = BackDirectCfg<R>::KPL, KC = 4 * KPL;
+    constexpr int NT = BackDirectCfg<R, V>::NT, IT = BackDirectCfg<R, V>::IT;
+    constexpr int KPL = BackDirectCfg<R, V>::KPL, KC = 4 * KPL;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    const int i = blockIdx.x * BP_BI + wave * 16 + lr;
-    const int n0 = blockIdx.y * BP_BN;
-    const int ic = min(i, mtxd - 1);
-    const bool iok = i < mtxd;
-    // dhd rows (bands) of this lane for the 4 row tiles, clamped
-    const cplx<R> *drow[4];
-    bool dok[4];
+    const int i0 = blockIdx.x * back_bi<R, V>() + wave * 16 * IT;
+    const int n0 = blockIdx.y * back_bn<R, V>();
+    int ic[IT];
+    bool iok[IT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
+    for (int q = 0; q < IT; q++) {
+        const int i = i0 + q * 16 + lr;
+        iok[q] = i < mtxd;
+        ic[q] = min(i, mtxd - 1);
+    }
+    // dhd rows (bands) of this lane for the NT row tiles, clamped
+    const cplx<R> *drow[NT];
+    bool dok[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
         const int n = n0 + t * 16 + lr;
         dok[t] = n < neig;
         drow[t] = dhd + (long)min(n, neig - 1) * nanl;
     }
-    v4 P1[4], P2[4], P3[4];
+    v4 P1[NT][IT], P2[NT][IT], P3[NT][IT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        P1[t] = v4{0, 0, 0, 0};
-        P2[t] = v4{0, 0, 0, 0};
-        P3[t] = v4{0, 0, 0, 0};
-    }
-    cplx<R> d0[4][KPL], d1[4][KPL], a0[KPL], a1[KPL];
-    auto load = [&](cplx<R>(&d)[4][KPL], cplx<R>(&av)[KPL], int ka) {
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < IT; q++) {
+            P1[t][q] = v4{0, 0, 0, 0};
+            P2[t][q] = v4{0, 0, 0, 0};
+            P3[t][q] = v4{0, 0, 0, 0};
+        }
+    cplx<R> d0[NT][KPL], d1[NT][KPL], a0[IT][KPL], a1[IT][KPL];
+    auto load = [&](cplx<R>(&d)[NT][KPL], cplx<R>(&av)[IT][KPL], int ka) {
 #pragma unroll
         for (int u = 0; u < KPL; u++) {
             const int a = ka + lg * KPL + u;
             const int acl = min(a, nanl - 1);
             const bool aok = a < nanl;
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < NT; t++) {
                 const cplx<R> v = drow[t][acl];
                 const bool ok = aok && dok[t];
                 d[t][u] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
             }
-            const cplx<R> w = anl[(long)acl * lda + ic];
-            const bool ok = aok && iok;
-            av[u] = mk<R>(ok ? w.x : (R)0, ok ? w.y : (R)0);
+#pragma unroll
+            for (int q = 0; q < IT; q++) {
+                const cplx<R> w = anl[(long)acl * lda + ic[q]];
+                const bool ok = aok && iok[q];
+                av[q][u] = mk<R>(ok ? w.x : (R)0, ok ? w.y : (R)0);
+            }
         }
     };
-    auto compute = [&](const cplx<R>(&d)[4][KPL], const cplx<R>(&av)[KPL]) {
+    auto compute = [&](const cplx<R>(&d)[NT][KPL], const cplx<R>(&av)[IT][KPL]) {
 #pragma unroll
         for (int u = 0; u < KPL; u++) {
-            const R cr = av[u].x, ci = av[u].y, cs = av[u].x + av[u].y;
+            R cr[IT], ci[IT], cs[IT];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                P1[t] = MF<R>::mma(d[t][u].x, cr, P1[t]);
-                P2[t] = MF<R>::mma(d[t][u].y, ci, P2[t]);
-                P3[t] = MF<R>::mma(d[t][u].x + d[t][u].y, cs, P3[t]);
+            for (int q = 0; q < IT; q++) {
+                cr[q] = av[q][u].x;
+                ci[q] = av[q][u].y;
+                cs[q] = av[q][u].x + av[q][u].y;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const R ds = d[t][u].x + d[t][u].y;
+#pragma unroll
+                for (int q = 0; q < IT; q++) {
+                    P1[t][q] = MF<R>::mma(d[t][u].x, cr[q], P1[t][q]);
+                    P2[t][q] = MF<R>::mma(d[t][u].y, ci[q], P2[t][q]);
+                    P3[t][q] = MF<R>::mma(ds, cs[q], P3[t][q]);
+                }
             }
         }
     };
@@ -609,14 +648,17 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_back_direct(const cplx<R> *
         if (more2) load(d0, a0, ka + 2 * KC);
         if (more1) compute(d1, a1);
     }
-    if (iok) {
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+    for (int q = 0; q < IT; q++) {
+        if (!iok[q]) continue;
+        const int i = i0 + q * 16 + lr;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int n = n0 + t * 16 + MF<R>::row(lg, r);
                 if (n < neig) {
-                    const R p1 = P1[t][r], p2 = P2[t][r], p3 = P3[t][r];
+                    const R p1 = P1[t][q][r], p2 = P2[t][q][r], p3 = P3[t][q][r];
                     double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
                     if (beta) {
                         const double2 o = hpsi[(long)n * ldh + i];

@@ -247,8 +247,8 @@ class Plan:
     def set_variant(self, prec, what, value):
         """Kernel choice for a precision: what = TUNE_PLANE (plane-stage kernel,
         0 LDS two-pass, 1..3 register shapes, -1 default), TUNE_PROJ
-        (projection GEMM, 0 LDS-staged, 1 direct loads), TUNE_BACK
-        (back-projection GEMM, same choices) or TUNE_SPLITK
+        (projection GEMM, 0 LDS-staged, 1..4 direct-load shapes), TUNE_BACK
+        (back-projection GEMM, 0 LDS-staged, 1..3 direct-load shapes) or TUNE_SPLITK
         (projection GEMM split-K target blocks, -1 default)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 

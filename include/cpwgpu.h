@@ -113,9 +113,11 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
  *                       GEMM (64..4096); -1 = default (256)
  *   CPWGPU_TUNE_PROJ    kernel of the projection GEMM: 0 = LDS-staged tiles,
- *                       1 = direct register loads; -1 = default (0)
+ *                       1..4 = direct register loads in four tile shapes;
+ *                       -1 = default (0)
  *   CPWGPU_TUNE_BACK    kernel of the back-projection GEMM: 0 = LDS-staged D,
- *                       1 = direct register loads; -1 = default (0) */
+ *                       1..3 = direct register loads in three tile shapes;
+ *                       -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2

@@ -77,7 +77,7 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
             assert plan.get_variant(prec, hk.TUNE_PLANE) == v
             for jv in (0, 1):
                 plan.set_variant(prec, hk.TUNE_PROJ, jv)
-                plan.set_variant(prec, hk.TUNE_BACK, jv)
+                plan.set_variant(prec, hk.TUNE_BACK, min(jv, 3))
                 h = _apply(torch, plan, pt, prec).cpu().numpy().T
                 assert rel(h, href) < tol, (name, prec, v, jv, rel(h, href))
     plan.close()
@@ -96,14 +96,14 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             plan.set_variant(prec, hk.TUNE_PLANE, v)
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, v)
-        for bv in (0, 1):
+        for bv in (0, 1, 2, 3):
             plan.set_variant(prec, hk.TUNE_BACK, bv)
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "back", bv)
         plan.set_variant(prec, hk.TUNE_BACK, -1)
-        for jv in (0, 1):
+        for jv in (0, 1, 2, 3, 4):
             plan.set_variant(prec, hk.TUNE_PROJ, jv)
-            for sk in (64, 128, 512, 1024, 4096):
+            for sk in (64, 512, 4096):
                 plan.set_variant(prec, hk.TUNE_SPLITK, sk)
                 h = _apply(torch, plan, pt, prec)
                 assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, jv, sk)
@@ -117,7 +117,7 @@ def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 2), (hk.TUNE_BACK, 2), (7, 0)):
+                        (hk.TUNE_PROJ, 5), (hk.TUNE_BACK, 4), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -139,7 +139,7 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         choice = plan.tune(pt, ht, prec)
         torch.cuda.synchronize()
         assert choice["plane"] in _variants(hk, plan, prec)
-        assert choice["proj"] in (0, 1) and choice["back"] in (0, 1)
+        assert choice["proj"] in (0, 1, 2, 3, 4) and choice["back"] in (0, 1, 2, 3)
         assert choice["splitk"] in (128, 256, 512, 1024)
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
         plan.set_variant(prec, hk.TUNE_PROJ, choice["proj"])
