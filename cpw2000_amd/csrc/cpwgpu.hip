@@ -339,7 +339,7 @@ struct LocalArgs {
     double2 *hpsi;
     long ldh;
     int nb;
-    int acc;  // col_fwd adds to hpsi (non-local term already there)
+    int mode;  // COLF_ACC / COLF_KIN: what the gather adds to local/N
 };
 
 template <typename R, int N>
@@ -360,7 +360,7 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
                            (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
                            (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
                            (const double *)p->ekpg.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb,
-                           p->inv_n, a.acc);
+                           p->inv_n, a.mode);
     } else {
         const size_t dyn =
             sizeof(cplx<R>) * ((size_t)N * plane_sub_stride(g.nx) + (size_t)plane_rows_fast<N>() * row_stride<N>());
@@ -532,7 +532,7 @@ int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
                            a.psi, a.ldp, a.hpsi, a.ldh, (const cplx<R> *)colbuf,
                            (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
                            (const int *)p->ent_cz.p, (const double *)p->ekpg.p, tw3, g, ntiles,
-                           p->inv_n, a.acc);
+                           p->inv_n, a.mode);
     }
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
@@ -608,17 +608,22 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
         const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_DIRECT ? p->back_var[ix] : 0;
         const cplx<R> *dh = (const cplx<R> *)p->dhd.p;
         double2 *hp = (double2 *)hpsi;
+        // the kinetic term ekpg * psi is added here (coalesced), not in the gather
+        const double2 *kp = (const double2 *)psi;
+        const double *ek = (const double *)p->ekpg.p;
 #define CPW_BD(V)                                                                                  \
     if (bv == V)                                                                                   \
         hipLaunchKernelGGL((k_back_direct<R, V>),                                                  \
                            dim3((g.mtxd + back_bi<R, V>() - 1) / back_bi<R, V>(),                  \
                                 (neig + back_bn<R, V>() - 1) / back_bn<R, V>()),                   \
-                           dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta);
+                           dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta, kp,   \
+                           ldp, ek);
         CPW_BD(1) CPW_BD(2) CPW_BD(3)
 #undef CPW_BD
         if (bv == 0)
             hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
-                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta);
+                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta,
+                               kp, ldp, ek);
     }
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
@@ -656,8 +661,10 @@ int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
     bool joined = join == nullptr;
     for (int b0 = 0; b0 < neig; b0 += nbc) {
         const int nb = std::min(nbc, neig - b0);
+        // with the non-local term the back-projection already wrote it and the
+        // kinetic term into hpsi; without it the gather adds the kinetic term
         const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
-                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, acc ? 1 : 0};
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, acc ? COLF_ACC : COLF_KIN};
         if (g.ncol > 0) {
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, s))) return st;
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, s))) return st;

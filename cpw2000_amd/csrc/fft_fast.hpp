@@ -118,14 +118,14 @@ __global__ __launch_bounds__(FT) void k_col_bwd_fast(
 }
 
 // ---------------------------------------------------------------------------
-// colbuf -> forward FFT along z -> hpsi = local/N + ekpg psi
+// colbuf -> forward FFT along z -> hpsi = local/N (+ ekpg psi) (+ hpsi)
 // ---------------------------------------------------------------------------
 template <typename R, int N3>
 __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     const double2 *__restrict__ psi, long ldp, double2 *__restrict__ hpsi, long ldh,
     const cplx<R> *__restrict__ colbuf, const int *__restrict__ col_ptr,
     const int *__restrict__ ent_i, const int *__restrict__ ent_cz, const double *__restrict__ ekpg,
-    const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb, double inv_n, int acc) {
+    const double2 *__restrict__ tw3, FftGeom g, int ntiles, int nb, double inv_n, int mode) {
     constexpr int CT = col_tile<N3>();
     constexpr int LS = N3 + 1;
     __shared__ cplx<R> tw[N3];
@@ -138,13 +138,13 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     const double2 *pb = psi + (long)band * ldp;
     double2 *hb = hpsi + (long)band * ldh;
     const int e0 = col_ptr[c0], e1 = col_ptr[c0 + nc];
-    // The first CF_PRE sphere entries of each thread: index, psi, ekpg and the
-    // non-local term in hpsi are loaded before the transform, so their two
-    // dependent global round trips overlap it.
+    // The first CF_PRE sphere entries of each thread: index, and the terms
+    // added to local/N (ekpg*psi and/or what hpsi already holds) are loaded
+    // before the transform, so their two dependent global round trips overlap it.
+    const bool kin = mode & COLF_KIN, acc = mode & COLF_ACC;
     constexpr int CF_PRE = 4;
     int pcz[CF_PRE], pi[CF_PRE];
-    double2 pp[CF_PRE], ph[CF_PRE];
-    double pk[CF_PRE];
+    double2 ph[CF_PRE];
 #pragma unroll
     for (int q = 0; q < CF_PRE; q++) {
         const int e = e0 + (int)threadIdx.x + q * FT;
@@ -154,9 +154,13 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     }
 #pragma unroll
     for (int q = 0; q < CF_PRE; q++) {
-        pp[q] = pb[pi[q]];
-        pk[q] = ekpg[pi[q]];
         ph[q] = acc ? hb[pi[q]] : make_double2(0.0, 0.0);
+        if (kin) {
+            const double2 p = pb[pi[q]];
+            const double ek = ekpg[pi[q]];
+            ph[q].x += ek * p.x;
+            ph[q].y += ek * p.y;
+        }
     }
     const cplx<R> *ib = colbuf + (long)band * N3 * g.ncolp + c0;
     for (int t = threadIdx.x; t < N3 * CT; t += FT) {
@@ -170,21 +174,20 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
     for (int q = 0; q < CF_PRE; q++) {
         if (e0 + (int)threadIdx.x + q * FT < e1) {
             const cplx<R> v = lines[((pcz[q] >> 12) - c0) * LS + (pcz[q] & 0xfff)];
-            double2 h;
-            h.x = (double)v.x * inv_n + pk[q] * pp[q].x + ph[q].x;
-            h.y = (double)v.y * inv_n + pk[q] * pp[q].y + ph[q].y;
-            hb[pi[q]] = h;
+            hb[pi[q]] = make_double2((double)v.x * inv_n + ph[q].x, (double)v.y * inv_n + ph[q].y);
         }
     }
     for (int e = e0 + threadIdx.x + CF_PRE * FT; e < e1; e += FT) {
         const int cz = ent_cz[e];
         const int i = ent_i[e];
         const cplx<R> v = lines[((cz >> 12) - c0) * LS + (cz & 0xfff)];
-        const double2 p = pb[i];
-        const double ek = ekpg[i];
-        double2 h;
-        h.x = (double)v.x * inv_n + ek * p.x;
-        h.y = (double)v.y * inv_n + ek * p.y;
+        double2 h = make_double2((double)v.x * inv_n, (double)v.y * inv_n);
+        if (kin) {
+            const double2 p = pb[i];
+            const double ek = ekpg[i];
+            h.x += ek * p.x;
+            h.y += ek * p.y;
+        }
         if (acc) {
             const double2 o = hb[i];
             h.x += o.x;

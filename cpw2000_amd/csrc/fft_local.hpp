@@ -30,6 +30,10 @@
 
 namespace cpwgpu {
 
+// what the gather of the forward column kernels adds to local/N
+constexpr int COLF_ACC = 1;  // the value already in hpsi (non-local + kinetic terms)
+constexpr int COLF_KIN = 2;  // ekpg * psi
+
 // Geometry of one k-point's FFT problem (built on the host).
 constexpr int MAXRAD = 12;
 struct FftGeom {
@@ -253,14 +257,15 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_plane(
 }
 
 // ---------------------------------------------------------------------------
-// k_col_fwd: colbuf -> forward FFT along z -> hpsi(G) = local/N + ekpg*psi
+// k_col_fwd: colbuf -> forward FFT along z -> hpsi(G) = local/N (+ ekpg*psi if
+// mode & COLF_KIN) (+ the term already in hpsi if mode & COLF_ACC)
 // ---------------------------------------------------------------------------
 template <typename R, int CT, int KMAX>
 __global__ __launch_bounds__(COL_THREADS) void k_col_fwd(
     const double2 *__restrict__ psi, long ldp, double2 *__restrict__ hpsi, long ldh,
     const cplx<R> *__restrict__ colbuf, const int *__restrict__ col_ptr,
     const int *__restrict__ ent_i, const int *__restrict__ ent_cz, const double *__restrict__ ekpg,
-    const double2 *__restrict__ tw3, FftGeom g, int ntiles, double inv_n, int acc) {
+    const double2 *__restrict__ tw3, FftGeom g, int ntiles, double inv_n, int mode) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n3 = g.n[2];
     const int ls = n3 + 1;
@@ -286,12 +291,14 @@ __global__ __launch_bounds__(COL_THREADS) void k_col_fwd(
         const int cz = ent_cz[e];
         const int i = ent_i[e];
         const cplx<R> v = lines[((cz >> 12) - c0) * ls + (cz & 0xfff)];
-        const double2 p = pb[i];
-        const double ek = ekpg[i];
-        double2 h;
-        h.x = (double)v.x * inv_n + ek * p.x;
-        h.y = (double)v.y * inv_n + ek * p.y;
-        if (acc) {  // the non-local term is already in hpsi
+        double2 h = make_double2((double)v.x * inv_n, (double)v.y * inv_n);
+        if (mode & COLF_KIN) {
+            const double2 p = pb[i];
+            const double ek = ekpg[i];
+            h.x += ek * p.x;
+            h.y += ek * p.y;
+        }
+        if (mode & COLF_ACC) {  // the non-local (and kinetic) term is already in hpsi
             const double2 o = hb[i];
             h.x += o.x;
             h.y += o.y;

@@ -423,7 +423,8 @@ constexpr int BP_BI = 64;  // plane waves per block (16 per wave)
 constexpr int BP_BN = 64;  // bands per block
 
 // ---------------------------------------------------------------------------
-// k_back: hpsi[i,n] = beta*hpsi[i,n] + sum_a anlga[i,a] dhd[a,n], beta 0 or 1
+// k_back: hpsi[i,n] = beta*hpsi[i,n] + sum_a anlga[i,a] dhd[a,n]
+// (+ ekpg[i] kpsi[i,n] when kpsi is given: the kinetic term), beta 0 or 1
 // computed as D^T = dhd^T anlga^T so that each MFMA output row is 16
 // consecutive plane waves (coalesced 256-byte stores of hpsi).
 // grid (ceil(mtxd/64), ceil(neig/64)); wave w owns plane waves i0+16w..+15
@@ -435,7 +436,10 @@ template <typename R>
 __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict__ anl, long lda,
                                                        const cplx<R> *__restrict__ dhd,
                                                        double2 *__restrict__ hpsi, long ldh,
-                                                       int mtxd, int nanl, int neig, int beta) {
+                                                       int mtxd, int nanl, int neig, int beta,
+                                                       const double2 *__restrict__ kpsi = nullptr,
+                                                       long ldk = 0,
+                                                       const double *__restrict__ ekpg = nullptr) {
     typedef typename MF<R>::v4 v4;
     constexpr int KC = GemmCfg<R>::KC, KCP = GemmCfg<R>::KCP;
     constexpr int KPL = KC / 4;             // projectors per lane group per chunk
@@ -526,6 +530,11 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
                 if (n < neig) {
                     const R p1 = P1[ni][r], p2 = P2[ni][r], p3 = P3[ni][r];
                     double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
+                    if (kpsi) {  // kinetic term in FP64 (coalesced rows of psi)
+                        const double2 p = kpsi[(long)n * ldk + i];
+                        h.x += ekpg[i] * p.x;
+                        h.y += ekpg[i] * p.y;
+                    }
                     if (beta) {
                         const double2 o = hpsi[(long)n * ldh + i];
                         h.x += o.x;
@@ -563,7 +572,8 @@ template <typename R, int V> constexpr int back_bn() { return 16 * BackDirectCfg
 template <typename R, int V>
 __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back_direct(
     const cplx<R> *__restrict__ anl, long lda, const cplx<R> *__restrict__ dhd,
-    double2 *__restrict__ hpsi, long ldh, int mtxd, int nanl, int neig, int beta) {
+    double2 *__restrict__ hpsi, long ldh, int mtxd, int nanl, int neig, int beta,
+    const double2 *__restrict__ kpsi, long ldk, const double *__restrict__ ekpg) {
     typedef typename MF<R>::v4 v4;
     constexpr int NT = BackDirectCfg<R, V>::NT, IT = BackDirectCfg<R, V>::IT;
     constexpr int KPL = BackDirectCfg<R, V>::KPL, KC = 4 * KPL;
@@ -660,6 +670,11 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
                 if (n < neig) {
                     const R p1 = P1[t][q][r], p2 = P2[t][q][r], p3 = P3[t][q][r];
                     double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
+                    if (kpsi) {  // kinetic term in FP64 (coalesced rows of psi)
+                        const double2 p = kpsi[(long)n * ldk + i];
+                        h.x += ekpg[i] * p.x;
+                        h.y += ekpg[i] * p.y;
+                    }
                     if (beta) {
                         const double2 o = hpsi[(long)n * ldh + i];
                         h.x += o.x;
