@@ -219,6 +219,46 @@ def bench_kpoints(torch, hk, synth, np, steps):
     return out
 
 
+def bench_ditsp(torch, hk, plan, d, nb, np):
+    """Device-resident iterative diagonalisation of the 64-atom cell: the
+    lowest nb bands to max residual^2 < epspsi^2 (epspsi = 1e-6), FP64 and
+    FP32-mixed; wall time of the whole call.  "cold": from the small-matrix
+    start.  "scf_step": from the converged bands of a potential 2% weaker,
+    as in a self-consistency step."""
+    mtxd = d["mtxd"]
+    out = {}
+    eis = {}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        psi = torch.zeros((nb, mtxd), dtype=torch.complex128, device="cuda")
+        hpsi = torch.zeros_like(psi)
+        plan.ditsp(psi, hpsi, prec=pid, icmax=2, epspsi=1e-6)  # warm-up (handles, buffers)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ei, ifail, st = plan.ditsp(psi, hpsi, prec=pid, icmax=200, epspsi=1e-6)
+        dt = time.perf_counter() - t0
+        eis[prec] = ei
+        out[prec] = {"cold": {"seconds": round(dt, 4), "ifail": ifail, **st}}
+        log(f"bench: ditsp {prec} cold: {dt * 1e3:.1f} ms, {st['iters']} iterations, "
+            f"{st['hpsi_products']} H|psi> products, ifail {ifail}")
+        # self-consistency step: previous bands of V, now V*1.02
+        plan.set_potential(np.asfortranarray(d["vscr"] * 0.98))
+        plan.ditsp(psi, hpsi, prec=pid, icmax=200, iguess=1, epspsi=1e-6)
+        plan.set_potential(d["vscr"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ei2, ifail, st = plan.ditsp(psi, hpsi, prec=pid, icmax=200, iguess=1, epspsi=1e-6)
+        dt = time.perf_counter() - t0
+        out[prec]["scf_step"] = {"seconds": round(dt, 4), "ifail": ifail, **st,
+                                 "eig_vs_cold_ha": float(np.abs(ei2 - ei).max())}
+        log(f"bench: ditsp {prec} scf step: {dt * 1e3:.1f} ms, {st['iters']} iterations, "
+            f"{st['hpsi_products']} H|psi> products, ifail {ifail}")
+        del psi, hpsi
+    out["eig_fp32_vs_fp64_ha"] = float(np.abs(eis["fp64"] - eis["fp32"]).max())
+    out["bands"] = nb
+    out["epspsi"] = 1e-6
+    return out
+
+
 def bench_cfg2(torch, hk, synth, np, steps):
     """216-atom Si (108^3 mesh, 864 projectors, 512 bands): throughput of both
     paths and the FP32-mixed error against FP64 (H psi norm and Ritz values of
@@ -267,6 +307,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kpoints", action="store_true", help="skip the 16 k-point graph section")
+    ap.add_argument("--no-ditsp", action="store_true",
+                    help="skip the iterative diagonalisation section")
     ap.add_argument("--extra", action="store_true",
                     help="also measure the 216-atom cell (108^3 mesh, 512 bands)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -383,6 +425,9 @@ def main():
     if not args.no_kpoints:
         log("bench: 16 k-points, eager loop vs one graph launch")
         out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))
+    if not args.no_ditsp:
+        log("bench: iterative diagonalisation on the device (128 bands)")
+        out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
     if args.extra:
         log("bench: 216-atom cell (108^3, 512 bands)")
         out["si216"] = bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))

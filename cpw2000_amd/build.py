@@ -1,7 +1,7 @@
 """Build recipe for the native libraries of cpw2000_amd (in-tree).
 
   libcpwgpu.so          HIP kernels + C ABI (include/cpwgpu.h), gfx950
-  libcpwgpu_fortran.so  Fortran drop-in hk_psi_c16 over the C ABI (flang)
+  libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16 and ditsp_c16 over the C ABI (flang)
 
 Run ``python -m cpw2000_amd.build``.
 """
@@ -39,15 +39,18 @@ def _run(cmd):
 def build_hip():
     srcs = [os.path.join(CSRC, "cpwgpu.hip")]
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs])
+    # rocSOLVER (zheevd, zpotrf) and rocBLAS (ztrsm) for the small dense
+    # eigenproblems of the iterative diagonalisation
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs,
+          "-lrocsolver", "-lrocblas"])
     os.replace(tmp, LIB)
 
 
 def build_fortran():
-    src = os.path.join(PKG, "fortran", "hk_psi_c16.f90")
+    srcs = [os.path.join(PKG, "fortran", f) for f in ("hk_psi_c16.f90", "ditsp_c16.f90")]
     tmp = FLIB + ".tmp"
     objdir = os.path.join(PKG, "fortran")
-    _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, src,
+    _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, *srcs,
           "-L", PKG, "-lcpwgpu", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FLIB)
 

@@ -6,6 +6,8 @@
 // band-batched FFT kernels (fft_local.hpp) and the non-local term
 // (Src/cpw_lib/hk_psi_nl_c16.f90) two MFMA GEMMs (nl_gemm.hpp).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -24,6 +26,7 @@
 #include "nl_gemm.hpp"
 #include "subspace.hpp"
 #include "forces.hpp"
+#include "ditsp.hpp"
 
 using namespace cpwgpu;
 
@@ -173,6 +176,7 @@ struct cpwgpu_plan_s {
     // device arrays
     DevBuf col_ptr, col_uy, ent_i, ent_cz, ekpg, tw[3];
     DevBuf v64, v32;          // V(r) in FP64 and FP32
+    double v0 = 0.0;          // mean of V(r) over the mesh = V(G=0), the local part of diag(H)
     DevBuf anl64, anl32, xnl; // projectors
     long lda = 0;             // leading dimension of anl64
     long lda32 = 0;           // of anl32: lda rounded up to even (16-byte aligned rows)
@@ -709,6 +713,17 @@ int set_potential(cpwgpu_plan_s *p, const double *vscr, int mxdscr) {
     int st = ensure_stream();
     if (st) return st;
     if ((st = upload(p->v64, vscr, p->ntot))) return st;
+    {
+        // V(G=0): mean over the n1*n2*n3 points (x beyond n1 is padding when id > n1)
+        double s = 0.0;
+        const int n1 = p->g.n[0], n2 = p->g.n[1], n3 = p->g.n[2], id = p->g.id;
+        for (int k3 = 0; k3 < n3; k3++)
+            for (int k2 = 0; k2 < n2; k2++) {
+                const double *row = vscr + ((size_t)k3 * n2 + k2) * id;
+                for (int k1 = 0; k1 < n1; k1++) s += row[k1];
+            }
+        p->v0 = s / ((double)n1 * n2 * n3);
+    }
     if ((st = p->v32.reserve(sizeof(float) * p->ntot))) return st;
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((p->ntot + 255) / 256)), dim3(256), 0, g_stream,
                        (const double *)p->v64.p, (float *)p->v32.p, (long)p->ntot);
@@ -739,7 +754,7 @@ struct DropIn {
     cpwgpu_plan_s *plan = nullptr;
     int key_ng = -1, key_mtxd = -1;
     int key_kmscr[7] = {0};
-    DevBuf psi, hpsi;
+    DevBuf psi, hpsi, hs, hd;
 } g_drop;
 
 int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
@@ -859,6 +874,417 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(eig_h, g_ss.eig.p, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return CPWGPU_OK;
+}
+
+// ---- block iterative eigensolver (device-resident ditsp) ----------------------
+// The lowest eigenpairs of the plan's H with everything on the device: the
+// small-matrix start of Src/ditsp/ditsp_c16_start.f90 (zheevd of the leading
+// block of H, its eigenvectors padded with zeros), then steps of a locally
+// optimal block preconditioned iteration: the Jacobi-relaxed residuals of the
+// vectors not yet converged (Src/omp/diag_rq_jac_c16.f90) and their previous
+// search directions are orthonormalised against the vectors, H is applied to
+// them, and a Rayleigh-Ritz step on [vectors, residuals, directions] keeps
+// the lowest ones, until max_n |H psi_n - e_n psi_n|^2 < epspsi^2 over the
+// wanted bands (the test of ditsp_c16.f90:375-377).  The reference relaxes
+// each band with a fixed step between Rayleigh-Ritz steps instead; its
+// convergence test, preconditioner and start are kept.  The
+// orthonormalisation is Gram-Schmidt against the vectors plus Cholesky-QR of
+// the new block (two passes each) instead of the reference's modified
+// Gram-Schmidt; the small dense eigenproblems go to rocSOLVER's zheevd.
+rocblas_handle g_blas = nullptr;
+
+#define BLASCHK(x)                                                                         \
+    do {                                                                                   \
+        rocblas_status s_ = (x);                                                           \
+        if (s_ != rocblas_status_success)                                                  \
+            return fail(CPWGPU_ERR_HIP, "%s: rocBLAS/rocSOLVER status %d (%s:%d)", #x,     \
+                        (int)s_, __FILE__, __LINE__);                                      \
+    } while (0)
+
+int ensure_blas(hipStream_t st) {
+    if (!g_blas) {
+        HIPCHK(hipSetDevice(g_device));
+        BLASCHK(rocblas_create_handle(&g_blas));
+    }
+    BLASCHK(rocblas_set_stream(g_blas, st));
+    return CPWGPU_OK;
+}
+
+struct DitspWs {
+    DevBuf xa, hxa, xb, hxb, pb, cpk, hd, m, g, g0, pm, w, e, info, res, act, d, mones;
+} g_dt;
+
+constexpr double DT_PIVOT = 1e-10;      // squared norm a unit column must keep outside the span
+                                        // of the columns before it
+constexpr double DT_SWITCH2 = 1e-10;    // FP32-mixed: residual^2 where FP64 H takes over
+constexpr int DT_MAX_SMALL = 4096;      // largest small matrix of the start
+constexpr unsigned long long DT_SEED = 0x243F6A8885A308D3ull;
+
+// out(na, nbc) = a^H b over mtxd rows on the matrix cores (split-K over the
+// plane waves, fixed-order reduction); row r of out is multiplied by scale[r]
+// when scale is given.  lower != 0 (a Hermitian result, na == nbc): only the
+// lower triangle is computed, the strict upper one is set to zero.
+int gram(const double2 *a, long lda, int na, const double2 *b, long ldb, int nbc, int mtxd,
+         const double *scale, double2 *out, hipStream_t st, int lower = 0) {
+    constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
+    const int tm = (na + BM - 1) / BM, tn = (nbc + BN - 1) / BN;
+    const int ksteps = (mtxd + BK - 1) / BK;
+    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+    nsl = (mtxd + kslice - 1) / kslice;
+    int rc;
+    if ((rc = g_ss.part.reserve(16 * (size_t)nsl * na * nbc))) return rc;
+    hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
+                       (const cplx<double> *)a, lda, b, ldb, (cplx<double> *)g_ss.part.p, mtxd, na,
+                       nbc, kslice, nullptr, lower);
+    launch_proj_reduce<double>((const cplx<double> *)g_ss.part.p, scale, (cplx<double> *)out, na,
+                               nbc, nsl, st, lower);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// y(:, 0:nc) = beta*y + x(:, 0:nk) c; c column-major (nk, nc) with ld nk; beta 0 or 1
+int rotate(const double2 *x, long ldx, int nk, const double2 *c, int nc, double2 *y, long ldy,
+           int mtxd, int beta, hipStream_t st) {
+    hipLaunchKernelGGL(k_back<double>, dim3((mtxd + BP_BI - 1) / BP_BI, (nc + BP_BN - 1) / BP_BN),
+                       dim3(GEMM_THREADS), 0, st, (const cplx<double> *)x, ldx,
+                       (const cplx<double> *)c, y, ldy, mtxd, nk, nc, beta);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+inline dim3 dt_grid(int rows, int cols) {
+    return dim3((unsigned)((rows + DT_THREADS - 1) / DT_THREADS), (unsigned)cols);
+}
+
+// Orthonormalises t(:, 0:m) (unit columns on entry) against the orthonormal
+// x(:, 0:nx) and among themselves: Gram-Schmidt against x, twice, then
+// Cholesky-QR of t, then once more of each.  In the first Cholesky pass the
+// columns from the first one whose squared norm outside the span of x and of
+// the columns before it is below DT_PIVOT are dropped; *kept is what is left.
+int orthonormalize(int mtxd, const double2 *x, long ldx, int nx, double2 *t, long ldt, int m,
+                   int *kept, hipStream_t st) {
+    int rc;
+    *kept = 0;
+    if (m <= 0) return CPWGPU_OK;
+    if ((rc = g_dt.pm.reserve(16 * (size_t)std::max(nx, 1) * m)) ||
+        (rc = g_dt.g.reserve(16 * (size_t)m * m)) || (rc = g_dt.g0.reserve(16 * (size_t)m * m)) ||
+        (rc = g_dt.info.reserve(64)) || (rc = g_dt.d.reserve(8 * (size_t)m)) ||
+        (rc = g_dt.mones.reserve(8 * (size_t)std::max(nx, 1))))
+        return rc;
+    double2 *P = (double2 *)g_dt.pm.p, *G = (double2 *)g_dt.g.p, *G0 = (double2 *)g_dt.g0.p;
+    int *dinfo = (int *)g_dt.info.p;
+    if (nx > 0)
+        hipLaunchKernelGGL(k_fill, dim3((nx + DT_THREADS - 1) / DT_THREADS), dim3(DT_THREADS), 0, st,
+                           (double *)g_dt.mones.p, nx, -1.0);
+    auto project = [&]() -> int {
+        // P = -x^H t;  t += x P
+        int r;
+        if ((r = gram(x, ldx, nx, t, ldt, m, mtxd, (const double *)g_dt.mones.p, P, st))) return r;
+        return rotate(x, ldx, nx, P, m, t, ldt, mtxd, 1, st);
+    };
+    const rocblas_double_complex one(1.0, 0.0);
+    for (int pass = 0; pass < 2 && m > 0; pass++) {
+        if (nx > 0) {
+            if ((rc = project())) return rc;
+            if (pass == 0 && (rc = project())) return rc;
+        }
+        const int ld = m;
+        if ((rc = gram(t, ldt, m, t, ldt, m, mtxd, nullptr, G, st, 1))) return rc;
+        HIPCHK(hipMemcpyAsync(G0, G, 16 * (size_t)ld * ld, hipMemcpyDeviceToDevice, st));
+        BLASCHK(rocsolver_zpotrf(g_blas, rocblas_fill_lower, m, (rocblas_double_complex *)G, ld, dinfo));
+        int info = 0;
+        HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof info, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (info > 0) {
+            // leading minor of order info not positive definite: keep the columns before it
+            m = info - 1;
+            if (m == 0) break;
+            HIPCHK(hipMemcpyAsync(G, G0, 16 * (size_t)ld * ld, hipMemcpyDeviceToDevice, st));
+            BLASCHK(rocsolver_zpotrf(g_blas, rocblas_fill_lower, m, (rocblas_double_complex *)G, ld, dinfo));
+            HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof info, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (info != 0) return fail(CPWGPU_ERR_NOTPD, "Cholesky of a leading block failed (%d)", info);
+        }
+        if (pass == 0) {
+            std::vector<double> d(m);
+            hipLaunchKernelGGL(k_diag_sq, dim3((m + DT_THREADS - 1) / DT_THREADS), dim3(DT_THREADS), 0, st,
+                               (const double2 *)G, (long)ld, m, (double *)g_dt.d.p);
+            HIPCHK(hipMemcpyAsync(d.data(), g_dt.d.p, 8 * (size_t)m, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < m; k++)
+                if (d[k] < DT_PIVOT) {
+                    m = k;
+                    break;
+                }
+            if (m == 0) break;
+        }
+        // t <- t L^-H
+        BLASCHK(rocblas_ztrsm(g_blas, rocblas_side_right, rocblas_fill_lower,
+                              rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, mtxd, m,
+                              &one, (const rocblas_double_complex *)G, ld, (rocblas_double_complex *)t,
+                              (int)ldt));
+    }
+    *kept = m;
+    return CPWGPU_OK;
+}
+
+// Rayleigh-Ritz on the orthonormal basis xa(:, 0:nt) with hxa = H xa (ld
+// mtxd): the lowest nk Ritz vectors go to xo and H times them to hxo (ld
+// mtxd); all nt Ritz values to g_dt.w (ascending), zheevd's info to
+// g_dt.info[1].  When po is given and nt > nk, po receives the part of the
+// Ritz vectors from the columns beyond nk, xa(:, nk:nt) C(nk:nt, 0:nk): the
+// search directions of the next step.
+int ritz_step(int mtxd, const double2 *xa, const double2 *hxa, int nt, int nk, double2 *xo,
+              double2 *hxo, double2 *po, hipStream_t st) {
+    int rc;
+    double2 *M = (double2 *)g_dt.m.p;
+    if ((rc = gram(xa, mtxd, nt, hxa, mtxd, nt, mtxd, nullptr, M, st, 1))) return rc;
+    BLASCHK(rocsolver_zheevd(g_blas, rocblas_evect_original, rocblas_fill_lower, nt,
+                             (rocblas_double_complex *)M, nt, (double *)g_dt.w.p, (double *)g_dt.e.p,
+                             (int *)g_dt.info.p + 1));
+    if ((rc = rotate(xa, mtxd, nt, M, nk, xo, mtxd, mtxd, 0, st))) return rc;
+    if ((rc = rotate(hxa, mtxd, nt, M, nk, hxo, mtxd, mtxd, 0, st))) return rc;
+    if (po && nt > nk) {
+        double2 *cp = (double2 *)g_dt.cpk.p;
+        hipLaunchKernelGGL(k_pack_rows, dt_grid(nt - nk, nk), dim3(DT_THREADS), 0, st, (const double2 *)M,
+                           (long)nt, nk, nt - nk, nk, cp);
+        if ((rc = rotate(xa + (size_t)nk * mtxd, mtxd, nt - nk, cp, nk, po, mtxd, mtxd, 0, st))) return rc;
+    }
+    return CPWGPU_OK;
+}
+
+// mtxds from the diagonal of H, the rule of Src/ALFA/size_mtxds.f90:60-85
+int size_mtxds(const std::vector<double> &hd, int neig) {
+    const int mtxd = (int)hd.size();
+    constexpr int NFULL = 250;
+    constexpr double EPS = 1e-6;
+    if (mtxd < NFULL || 3 * neig + 20 >= mtxd) return mtxd;
+    const double href = std::fabs(2 * hd[neig - 1] - hd[0]) + EPS;
+    for (int i = 3 * neig + 20; i <= mtxd - 1; i++)  // 1-based i as in the reference
+        if (hd[i - 1] > href && hd[i] - hd[i - 1] > EPS) return i;
+    return mtxd;
+}
+
+int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi, int neig, int mtxds,
+               const double2 *hamsm, long ldhs, const double *hdiag_in, double2 *psi, long ldp,
+               double2 *hpsi, long ldh, double *ei, int *ifail, int *stats, hipStream_t st) {
+    const int mtxd = p->mtxd;
+    int rc;
+    if (!p->v64.p) return fail(CPWGPU_ERR_ARG, "the plan has no potential");
+    if ((rc = ensure_blas(st))) return rc;
+    long ihkp = 0;
+    auto happly = [&](int pr, const double2 *x, double2 *hx, int n) -> int {
+        ihkp += n;
+        return pr == CPWGPU_PREC_FP32
+                   ? apply_t<float>(p, n, (const double *)x, mtxd, (double *)hx, mtxd, st)
+                   : apply_t<double>(p, n, (const double *)x, mtxd, (double *)hx, mtxd, st);
+    };
+    const int nbuf = std::min(mtxd - neig, std::max(8, neig / 8));
+    const int nb = neig + nbuf;
+    // diagonal of H (preconditioner and the choice of mtxds)
+    if ((rc = g_dt.hd.reserve(8 * (size_t)mtxd))) return rc;
+    double *hd = (double *)g_dt.hd.p;
+    if (hdiag_in) {
+        HIPCHK(hipMemcpyAsync(hd, hdiag_in, 8 * (size_t)mtxd, hipMemcpyDeviceToDevice, st));
+    } else {
+        hipLaunchKernelGGL(k_hdiag, dim3((mtxd + DT_THREADS - 1) / DT_THREADS), dim3(DT_THREADS), 0, st,
+                           (const double *)p->ekpg.p, (const double2 *)p->anl64.p, p->lda,
+                           (const double *)p->xnl.p, p->nanl, p->v0, hd, mtxd);
+        HIPCHK(hipGetLastError());
+    }
+    if (mtxds <= 0) {
+        std::vector<double> h(mtxd);
+        HIPCHK(hipMemcpyAsync(h.data(), hd, 8 * (size_t)mtxd, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        mtxds = size_mtxds(h, neig);
+    }
+    mtxds = std::min(mtxds, mtxd);
+    if (neig > mtxds) return fail(CPWGPU_ERR_ARG, "neig = %d is greater than mtxds = %d", neig, mtxds);
+    const bool full = mtxds >= mtxd;
+    const bool small_start = full || iguess != 1;
+    if (small_start && mtxds > DT_MAX_SMALL)
+        return fail(CPWGPU_ERR_SIZE, "small matrix of %d > %d rows", mtxds, DT_MAX_SMALL);
+    const int ms = small_start ? mtxds : 0;
+    const int cap = std::max(3 * nb, small_start && !hamsm ? ms : 0);
+    const int mdim = std::max(3 * nb, ms);
+    const size_t col = 16 * (size_t)mtxd;
+    if ((rc = g_dt.xa.reserve(col * cap)) || (rc = g_dt.hxa.reserve(col * cap)) ||
+        (rc = g_dt.xb.reserve(col * nb)) || (rc = g_dt.hxb.reserve(col * nb)) ||
+        (rc = g_dt.pb.reserve(col * nb)) || (rc = g_dt.cpk.reserve(16 * (size_t)2 * nb * nb)) ||
+        (rc = g_dt.m.reserve(16 * (size_t)mdim * mdim)) || (rc = g_dt.w.reserve(8 * (size_t)mdim)) ||
+        (rc = g_dt.e.reserve(8 * (size_t)mdim)) || (rc = g_dt.info.reserve(64)) ||
+        (rc = g_dt.res.reserve(8 * (size_t)nb)) || (rc = g_dt.act.reserve(4 * (size_t)nb)))
+        return rc;
+    double2 *X = (double2 *)g_dt.xa.p, *HX = (double2 *)g_dt.hxa.p;
+    double2 *XB = (double2 *)g_dt.xb.p, *HXB = (double2 *)g_dt.hxb.p, *PB = (double2 *)g_dt.pb.p;
+    double2 *M = (double2 *)g_dt.m.p;
+    double *W = (double *)g_dt.w.p;
+    int *dinfo = (int *)g_dt.info.p;
+    HIPCHK(hipMemsetAsync(dinfo, 0, 64, st));
+    int refine_at = -1, iters = 0;
+    auto finish = [&](int conv, double xmax) {
+        if (ifail) {
+            int f = 0;
+            if (!conv) {
+                f = 1 + (int)(-std::log10(xmax));  // Src/ditsp/ditsp_c16.f90:408-409
+                if (f == 0) f = 1;
+            }
+            *ifail = f;
+        }
+        if (stats) {
+            stats[0] = iters;
+            stats[1] = (int)ihkp;
+            stats[2] = nb;
+            stats[3] = mtxds;
+            stats[4] = refine_at;
+        }
+    };
+    if (small_start) {
+        // small matrix: given, or the leading block of H from its action on unit vectors
+        if (hamsm) {
+            HIPCHK(hipMemcpy2DAsync(M, 16 * (size_t)ms, hamsm, 16 * (size_t)ldhs, 16 * (size_t)ms, ms,
+                                    hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL(k_unit_cols, dt_grid(mtxd, ms), dim3(DT_THREADS), 0, st, X, (long)mtxd, mtxd, ms);
+            if ((rc = happly(CPWGPU_PREC_FP64, X, HX, ms))) return rc;
+            hipLaunchKernelGGL(k_leading_block, dt_grid(ms, ms), dim3(DT_THREADS), 0, st,
+                               (const double2 *)HX, (long)mtxd, ms, M);
+        }
+        BLASCHK(rocsolver_zheevd(g_blas, rocblas_evect_original, rocblas_fill_lower, ms,
+                                 (rocblas_double_complex *)M, ms, W, (double *)g_dt.e.p, dinfo + 1));
+        int info = 0;
+        HIPCHK(hipMemcpyAsync(&info, dinfo + 1, sizeof info, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (info) return fail(CPWGPU_ERR_HIP, "zheevd of the small matrix failed (info %d)", info);
+        if (full) {
+            // the whole H was diagonalised (Src/ditsp/ditsp_c16.f90:218-245)
+            hipLaunchKernelGGL(k_pad_rows, dt_grid(mtxd, neig), dim3(DT_THREADS), 0, st, (const double2 *)M,
+                               (long)ms, ms, X, (long)mtxd, mtxd, neig);
+            if ((rc = happly(CPWGPU_PREC_FP64, X, HX, neig))) return rc;
+            HIPCHK(hipMemcpy2DAsync(psi, 16 * (size_t)ldp, X, col, col, neig, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)ldh, HX, col, col, neig, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(ei, W, 8 * (size_t)neig, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            finish(1, 0.0);
+            return CPWGPU_OK;
+        }
+        const int nu = std::min(nb, ms);
+        hipLaunchKernelGGL(k_pad_rows, dt_grid(mtxd, nu), dim3(DT_THREADS), 0, st, (const double2 *)M,
+                           (long)ms, ms, X, (long)mtxd, mtxd, nu);
+        if (nu < nb) {
+            hipLaunchKernelGGL(k_random_cols, dt_grid(mtxd, nb - nu), dim3(DT_THREADS), 0, st, X, (long)mtxd,
+                               mtxd, nu, nb - nu, (const double *)p->ekpg.p, DT_SEED);
+            hipLaunchKernelGGL(k_normalize_cols, dim3(nb - nu), dim3(DT_THREADS), 0, st, X + (size_t)nu * mtxd,
+                               (long)mtxd, mtxd);
+            int kept = 0;
+            if ((rc = orthonormalize(mtxd, X, mtxd, nu, X + (size_t)nu * mtxd, mtxd, nb - nu, &kept, st)))
+                return rc;
+            if (kept < nb - nu) return fail(CPWGPU_ERR_NOTPD, "start vectors are linearly dependent");
+        }
+    } else {
+        // neig guesses from the caller, buffer vectors random
+        HIPCHK(hipMemcpy2DAsync(X, col, psi, 16 * (size_t)ldp, col, neig, hipMemcpyDeviceToDevice, st));
+        if (nb > neig)
+            hipLaunchKernelGGL(k_random_cols, dt_grid(mtxd, nb - neig), dim3(DT_THREADS), 0, st, X,
+                               (long)mtxd, mtxd, neig, nb - neig, (const double *)p->ekpg.p, DT_SEED);
+        hipLaunchKernelGGL(k_normalize_cols, dim3(nb), dim3(DT_THREADS), 0, st, X, (long)mtxd, mtxd);
+        int kept = 0;
+        if ((rc = orthonormalize(mtxd, X, mtxd, 0, X, mtxd, nb, &kept, st))) return rc;
+        if (kept < nb) return fail(CPWGPU_ERR_NOTPD, "guess vectors are linearly dependent");
+    }
+    int pc = prec;
+    bool refined = prec == CPWGPU_PREC_FP64;
+    bool havep = false;
+    auto ritz_nb = [&](int nt) -> int {
+        int r = ritz_step(mtxd, X, HX, nt, nb, XB, HXB, nt > nb ? PB : nullptr, st);
+        havep = nt > nb;
+        if (r) return r;
+        HIPCHK(hipMemcpyAsync(X, XB, col * nb, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(HX, HXB, col * nb, hipMemcpyDeviceToDevice, st));
+        return CPWGPU_OK;
+    };
+    if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_nb(nb))) return rc;
+    const double eps2 = epspsi * epspsi;
+    const double sw2 = std::max(eps2, DT_SWITCH2);
+    // CPWGPU_DITSP_TRACE=1: one line per iteration on stderr (the ipr > 3
+    // diagnostics of Src/ditsp/ditsp_c16.f90:306-336)
+    static const bool trace = [] {
+        const char *e = getenv("CPWGPU_DITSP_TRACE");
+        return e && *e && *e != '0';
+    }();
+    std::vector<double> res(nb), wv(nb);
+    std::vector<int> act;
+    act.reserve(nb);
+    double xmax = 0.0, xprev = HUGE_VAL;
+    bool conv = false;
+    for (int ic = 0;; ic++) {
+        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, (const double2 *)X, (long)mtxd,
+                           (const double2 *)HX, (long)mtxd, (const double *)W, (double *)g_dt.res.p, mtxd);
+        HIPCHK(hipGetLastError());
+        int info = 0;
+        HIPCHK(hipMemcpyAsync(res.data(), g_dt.res.p, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(wv.data(), W, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&info, dinfo + 1, sizeof info, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (info) return fail(CPWGPU_ERR_HIP, "zheevd of the subspace failed (info %d)", info);
+        iters = ic;
+        xmax = *std::max_element(res.begin(), res.begin() + neig);
+        if (trace)
+            fprintf(stderr, "ditsp: ic %4d  max res^2 %.3e  e[0] %.10f  e[neig-1] %.10f  %s\n", ic, xmax, wv[0],
+                    wv[neig - 1], refined ? "fp64" : "fp32");
+        if (!refined) {
+            // the FP32 products stop helping near their rounding level: switch
+            // there, or once the residual stalls below 1e-6
+            if (xmax < sw2 || (xmax < 1e-6 && xmax > 0.9 * xprev)) {
+                // FP32-mixed: FP64 H on the current vectors, then FP64 iterations
+                refined = true;
+                refine_at = ic;
+                pc = CPWGPU_PREC_FP64;
+                if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_nb(nb))) return rc;
+                xprev = HUGE_VAL;
+                continue;
+            }
+        } else if (xmax < eps2) {
+            conv = true;
+            break;
+        }
+        if (ic >= icmax) break;
+        // active vectors, largest residual first
+        const double thr = 0.01 * (refined ? eps2 : sw2);
+        act.clear();
+        for (int n = 0; n < nb; n++)
+            if (res[n] > thr) act.push_back(n);
+        std::stable_sort(act.begin(), act.end(), [&](int a, int b) { return res[a] > res[b]; });
+        const int m = (int)act.size();
+        if (m == 0) break;
+        HIPCHK(hipMemcpyAsync(g_dt.act.p, act.data(), 4 * (size_t)m, hipMemcpyHostToDevice, st));
+        double2 *T = X + (size_t)nb * mtxd, *HT = HX + (size_t)nb * mtxd;
+        hipLaunchKernelGGL(k_precondition, dt_grid(mtxd, m), dim3(DT_THREADS), 0, st, (const double2 *)X,
+                           (long)mtxd, (const double2 *)HX, (long)mtxd, (const double *)W, (const double *)hd,
+                           (const int *)g_dt.act.p, T, (long)mtxd, mtxd, m);
+        // previous search directions of the active vectors (the third block
+        // of a locally optimal block preconditioned iteration)
+        int mt = m;
+        if (havep) {
+            hipLaunchKernelGGL(k_gather_cols, dt_grid(mtxd, m), dim3(DT_THREADS), 0, st, (const double2 *)PB,
+                               (long)mtxd, (const int *)g_dt.act.p, T + (size_t)m * mtxd, (long)mtxd, mtxd, m);
+            mt = 2 * m;
+        }
+        hipLaunchKernelGGL(k_normalize_cols, dim3(mt), dim3(DT_THREADS), 0, st, T, (long)mtxd, mtxd);
+        int kept = 0;
+        if ((rc = orthonormalize(mtxd, X, mtxd, nb, T, mtxd, mt, &kept, st))) return rc;
+        if (trace) fprintf(stderr, "ditsp:        active %d  new directions %d of %d\n", m, kept, mt);
+        if (kept == 0) break;  // no direction left outside the current vectors
+        if ((rc = happly(pc, T, HT, kept)) || (rc = ritz_nb(nb + kept))) return rc;
+        xprev = xmax;
+    }
+    HIPCHK(hipMemcpy2DAsync(psi, 16 * (size_t)ldp, X, col, col, neig, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)ldh, HX, col, col, neig, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::copy(wv.begin(), wv.begin() + neig, ei);
+    finish(conv, xmax);
     return CPWGPU_OK;
 }
 
@@ -1169,6 +1595,17 @@ int cpwgpu_finalize(void) {
     g_drop.key_mtxd = -1;
     g_drop.psi.release();
     g_drop.hpsi.release();
+    g_drop.hs.release();
+    g_drop.hd.release();
+    if (g_blas) {
+        if (g_stream) HIPCHK(hipStreamSynchronize(g_stream));
+        rocblas_destroy_handle(g_blas);
+        g_blas = nullptr;
+    }
+    for (DevBuf *b : {&g_dt.xa, &g_dt.hxa, &g_dt.xb, &g_dt.hxb, &g_dt.pb, &g_dt.cpk, &g_dt.hd, &g_dt.m,
+                      &g_dt.g, &g_dt.g0,
+                      &g_dt.pm, &g_dt.w, &g_dt.e, &g_dt.info, &g_dt.res, &g_dt.act, &g_dt.d, &g_dt.mones})
+        b->release();
     if (g_aux) {
         HIPCHK(hipStreamSynchronize(g_aux));
         HIPCHK(hipStreamDestroy(g_aux));
@@ -1516,6 +1953,92 @@ int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, doubl
     HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)*mxddim, g_drop.hpsi.p, colb, colb, *neig,
                             hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_ditsp(cpwgpu_plan_t plan, int prec, int icmax, int iguess, double epspsi, int neig,
+                      int mtxds, const double *hamsm_dev, long ldhs, const double *hdiag_dev,
+                      double *psi_dev, long ldp, double *hpsi_dev, long ldh, double *ei, int *ifail,
+                      int *stats, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    if (ifail) *ifail = 0;
+    if (neig < 0 || neig > plan->mtxd) return fail(CPWGPU_ERR_ARG, "neig = %d, mtxd = %d", neig, plan->mtxd);
+    if (neig == 0) return CPWGPU_OK;
+    if (!psi_dev || !hpsi_dev || !ei || ldp < plan->mtxd || ldh < plan->mtxd)
+        return fail(CPWGPU_ERR_ARG, "bad psi/hpsi/ei arguments");
+    if (icmax < 0 || !(epspsi > 0.0)) return fail(CPWGPU_ERR_ARG, "icmax = %d, epspsi = %g", icmax, epspsi);
+    if (hamsm_dev && (mtxds < 1 || ldhs < mtxds))
+        return fail(CPWGPU_ERR_ARG, "hamsm given with mtxds = %d, ldhs = %ld", mtxds, ldhs);
+    int st = ensure_stream();
+    if (st) return st;
+    return plan_ditsp(plan, prec, icmax, iguess, epspsi, neig, mtxds, (const double2 *)hamsm_dev, ldhs, hdiag_dev,
+                      (double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, ei, ifail, stats, (hipStream_t)stream);
+}
+
+int cpwgpu_ditsp_c16(const int *ipr, int *ifail, const int *icmax, const int *iguess, const double *epspsi,
+                     const int *lnewanl, int *neig, const int *mtxd, const int *mtxds, double *psi,
+                     double *hpsi, double *ei, const double *ekpg, const int *isort, const double *vscr,
+                     const int *kmscr, const int *ng, const int *kgv, const double *anlga, const double *xnlkb,
+                     const int *nanl, const double *hamsm, const double *hdiag, const int *mxddim,
+                     const int *mxdsml, const int *mxdbnd, const int *mxdgve, const int *mxdscr,
+                     const int *mxdanl) {
+    (void)lnewanl;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ipr || !ifail || !icmax || !iguess || !epspsi || !neig || !mtxd || !mtxds || !kmscr || !ng || !nanl ||
+        !mxddim || !mxdsml || !mxdbnd || !mxdgve || !mxdscr || !mxdanl)
+        return fail(CPWGPU_ERR_ARG, "null scalar argument");
+    *ifail = 0;
+    // the argument checks of Src/ditsp/ditsp_c16.f90:151-212, in the same order
+    if (*neig < 1) {
+        printf("\n   WARNING:   ditsp_c16 called with number of eigenvalues = %8d\n", *neig);
+        return CPWGPU_OK;
+    }
+    if (*neig > *mtxds) return fail(CPWGPU_ERR_ARG, "neig = %d is greater than mtxds = %d", *neig, *mtxds);
+    if (*neig > *mxdbnd) return fail(CPWGPU_ERR_ARG, "neig = %d is greater than mxdbnd = %d", *neig, *mxdbnd);
+    if (3 * *neig > *mxdsml)
+        return fail(CPWGPU_ERR_ARG, "3*neig = %d is greater than mxdsml = %d", 3 * *neig, *mxdsml);
+    if (*mtxds > *mxdsml) return fail(CPWGPU_ERR_ARG, "mtxds = %d is greater than mxdsml = %d", *mtxds, *mxdsml);
+    if (*mtxd > *mxddim) return fail(CPWGPU_ERR_MTXD, "mtxd = %d is greater than mxddim = %d", *mtxd, *mxddim);
+    if (*nanl > *mxdanl) return fail(CPWGPU_ERR_ARG, "nanl = %d is greater than mxdanl = %d", *nanl, *mxdanl);
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d is greater than mxdgve = %d", *ng, *mxdgve);
+    if (!psi || !hpsi || !ei || !ekpg || !isort || !vscr || !kgv || !hamsm || !hdiag)
+        return fail(CPWGPU_ERR_ARG, "null array argument");
+    if (*mtxds > DT_MAX_SMALL) return fail(CPWGPU_ERR_SIZE, "mtxds = %d > %d", *mtxds, DT_MAX_SMALL);
+    cpwgpu_plan_s *p = nullptr;
+    int st = drop_in_plan(*mtxd, *ng, kgv, ekpg, isort, kmscr, *mxddim, *mxdgve, &p);
+    if (st) return st;
+    if ((st = set_potential(p, vscr, *mxdscr))) return st;
+    if ((st = set_projectors_host(p, anlga, xnlkb, *nanl, *mxddim))) return st;
+    const int n = *neig, m = *mtxd, ms = *mtxds;
+    const size_t colb = 16 * (size_t)m;
+    if ((st = g_drop.psi.reserve(colb * n)) || (st = g_drop.hpsi.reserve(colb * n)) ||
+        (st = g_drop.hs.reserve(16 * (size_t)ms * ms)) || (st = g_drop.hd.reserve(8 * (size_t)m)))
+        return st;
+    if (*iguess == 1)
+        HIPCHK(hipMemcpy2DAsync(g_drop.psi.p, colb, psi, 16 * (size_t)*mxddim, colb, n, hipMemcpyHostToDevice,
+                                g_stream));
+    HIPCHK(hipMemcpy2DAsync(g_drop.hs.p, 16 * (size_t)ms, hamsm, 16 * (size_t)*mxdsml, 16 * (size_t)ms, ms,
+                            hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(g_drop.hd.p, hdiag, 8 * (size_t)m, hipMemcpyHostToDevice, g_stream));
+    int stats[5] = {0, 0, 0, 0, -1};
+    double2 *dpsi = (double2 *)g_drop.psi.p, *dhpsi = (double2 *)g_drop.hpsi.p;
+    if ((st = plan_ditsp(p, g_prec, *icmax, *iguess, *epspsi, n, ms, (const double2 *)g_drop.hs.p, ms,
+                         (const double *)g_drop.hd.p, dpsi, m, dhpsi, m, ei, ifail, stats, g_stream)))
+        return st;
+    HIPCHK(hipMemcpy2DAsync(psi, 16 * (size_t)*mxddim, dpsi, colb, colb, n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)*mxddim, dhpsi, colb, colb, n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    // messages of Src/ditsp/ditsp_c16.f90:386-419
+    if (ms >= m) {
+        if (*ipr > 1) printf("\n  Using standard diagonalization  mtxd = %8d\n\n", m);
+    } else if (*ifail == 0) {
+        if (*ipr > 1) printf("\n  Successful diagonalization after%8d matrix-vector multiplications\n\n", stats[1]);
+    } else if (*ipr > 0) {
+        printf("\n  Unsuccessful diagonalization after%8d matrix-vector multiplications (ifail = %d)\n\n",
+               stats[1], *ifail);
+    }
     return CPWGPU_OK;
 }
 

@@ -79,7 +79,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
                                                        const double2 *__restrict__ psi, long ldp,
                                                        cplx<R> *__restrict__ part, int mtxd,
                                                        int nanl, int neig, int kslice,
-                                                       const double *__restrict__ wrow = nullptr) {
+                                                       const double *__restrict__ wrow = nullptr,
+                                                       int lower = 0) {
     typedef typename MF<R>::v4 v4;
     constexpr int WM = GemmCfg<R>::WM, WN = GemmCfg<R>::WN;
     constexpr int BK = GemmCfg<R>::BK, KP = GemmCfg<R>::KP;
@@ -90,6 +91,9 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
     __shared__ __align__(16) R sA[2][2][BM][KP];  // [buf][re/im][a][k]
     __shared__ __align__(16) R sB[2][2][BN][KP];  // [buf][re/im][n][k]
     const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    // lower != 0: only the lower triangle (a >= n) of a Hermitian Gram matrix
+    // is wanted; tiles wholly above the diagonal are skipped (whole block)
+    if (lower && a0 + BM <= n0) return;
     const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -374,7 +378,8 @@ template <typename R>
 __global__ __launch_bounds__(RED_OUT * RED_GRP) void k_proj_reduce(const cplx<R> *__restrict__ part,
                                                                   const double *__restrict__ xnl,
                                                                   cplx<R> *__restrict__ dhd,
-                                                                  int nanl, int neig, int nslice) {
+                                                                  int nanl, int neig, int nslice,
+                                                                  int lower = 0) {
     __shared__ cplx<R> red[RED_GRP][RED_OUT];
     const int o = threadIdx.x % RED_OUT, grp = threadIdx.x / RED_OUT;
     const long tot = (long)nanl * neig;
@@ -405,18 +410,20 @@ __global__ __launch_bounds__(RED_OUT * RED_GRP) void k_proj_reduce(const cplx<R>
             i += red[g2][o].y;
         }
         const R x = xnl ? (R)xnl[t % nanl] : (R)1;  // no scaling for plain Gram matrices
-        dhd[t] = mk<R>(x * r, x * i);
+        // lower: the strict upper triangle (partials of skipped tiles) is zero
+        const bool up = lower && (t % nanl) < (t / nanl);
+        dhd[t] = up ? mk<R>((R)0, (R)0) : mk<R>(x * r, x * i);
     }
 }
 
 // Launches k_proj_reduce over nanl * neig outputs (nothing to do when empty).
 template <typename R>
 inline void launch_proj_reduce(const cplx<R> *part, const double *xnl, cplx<R> *dhd, int nanl,
-                               int neig, int nslice, hipStream_t s) {
+                               int neig, int nslice, hipStream_t s, int lower = 0) {
     const long tot = (long)nanl * neig;
     if (tot <= 0) return;
     hipLaunchKernelGGL(k_proj_reduce<R>, dim3((unsigned)((tot + RED_OUT - 1) / RED_OUT)),
-                       dim3(RED_OUT * RED_GRP), 0, s, part, xnl, dhd, nanl, neig, nslice);
+                       dim3(RED_OUT * RED_GRP), 0, s, part, xnl, dhd, nanl, neig, nslice, lower);
 }
 
 constexpr int BP_BI = 64;  // plane waves per block (16 per wave)

@@ -48,6 +48,7 @@ EXPORTS = (
     "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz", "cpwgpu_plan_forces",
     "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
     "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune",
+    "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
@@ -104,6 +105,8 @@ def lib():
         L.cpwgpu_plan_get_variant.argtypes = [vp, C.c_int, C.c_int, ip]
         L.cpwgpu_plan_tune.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
+        L.cpwgpu_plan_ditsp.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                        dp, C.c_long, dp, dp, C.c_long, dp, C.c_long, dp, ip, ip, vp]
         _lib = L
     return _lib
 
@@ -152,6 +155,37 @@ def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr
         _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr))
     _check(st)
     return hpsi
+
+
+def ditsp_c16(ipr, icmax, iguess, epspsi, lnewanl, neig, mtxd, mtxds, psi, hpsi, ei, ekpg,
+              isort, vscr, kmscr, ng, kgv, anlga, xnlkb, nanl, hamsm, hdiag, mxddim, mxdsml,
+              mxdbnd, mxdgve, mxdscr, mxdanl):
+    """Iterative diagonalisation with the arguments of Src/ditsp/ditsp_c16.f90:20-27
+    (ifail, the reference's first output, is returned instead of passed).
+
+    Arrays are numpy, Fortran layout: psi/hpsi complex128 (mxddim, >=neig)
+    (psi holds guesses when iguess == 1), ei float64 (>=neig), hamsm
+    complex128 (mxdsml, mxdsml) holding the leading mtxds x mtxds block of H,
+    hdiag float64 (mxddim).  psi, hpsi and ei are written in place."""
+    for name, arr, dt in (("psi", psi, np.complex128), ("hpsi", hpsi, np.complex128),
+                          ("ei", ei, np.float64), ("kgv", kgv, np.int32),
+                          ("ekpg", ekpg, np.float64), ("isort", isort, np.int32),
+                          ("vscr", vscr, np.float64), ("kmscr", kmscr, np.int32),
+                          ("anlga", anlga, np.complex128), ("xnlkb", xnlkb, np.float64),
+                          ("hamsm", hamsm, np.complex128), ("hdiag", hdiag, np.float64)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    ifail = C.c_int(0)
+    lnew = C.c_int(1 if lnewanl else 0)
+    nref = C.c_int(int(neig))
+    eps = C.c_double(float(epspsi))
+    _check(lib().cpwgpu_ditsp_c16(
+        _iref(ipr), C.byref(ifail), _iref(icmax), _iref(iguess), C.byref(eps), C.byref(lnew),
+        C.byref(nref), _iref(mtxd), _iref(mtxds), _p(psi), _p(hpsi), _p(ei), _p(ekpg), _p(isort),
+        _p(vscr), _p(kmscr), _iref(ng), _p(kgv), _p(anlga), _p(xnlkb), _iref(nanl), _p(hamsm),
+        _p(hdiag), _iref(mxddim), _iref(mxdsml), _iref(mxdbnd), _iref(mxdgve), _iref(mxdscr),
+        _iref(mxdanl)))
+    return ifail.value
 
 
 def hk_psi_c16_dict(d, psi=None, hpsi=None, neig=None):
@@ -302,6 +336,32 @@ class Plan:
                                        _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
                                        int(prec), C.c_void_p(stream)))
         return hpsi_t
+
+    def ditsp(self, psi_t, hpsi_t, prec=PREC_FP64, icmax=100, iguess=0, epspsi=1e-6, mtxds=0,
+              hamsm_t=None, hdiag_t=None, stream=None):
+        """Lowest eigenpairs of H (cpwgpu_plan_ditsp): psi_t / hpsi_t are
+        complex128 device tensors (neig, ld); psi_t holds guesses when iguess
+        == 1 and gets the eigenvectors, hpsi_t H times them.  Returns (ei,
+        ifail, stats) with stats = dict(iters, hpsi_products, nvec, mtxds,
+        fp64_from)."""
+        import torch
+
+        neig = psi_t.shape[0]
+        ei = np.zeros(neig)
+        ifail = C.c_int(0)
+        stats = (C.c_int * 5)()
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        hs = None if hamsm_t is None else C.c_void_p(hamsm_t.data_ptr())
+        ldhs = 0 if hamsm_t is None else _ld(hamsm_t)
+        hd = None if hdiag_t is None else C.c_void_p(hdiag_t.data_ptr())
+        _check(lib().cpwgpu_plan_ditsp(
+            self._h, int(prec), int(icmax), int(iguess), float(epspsi), int(neig), int(mtxds), hs,
+            ldhs, hd, C.c_void_p(psi_t.data_ptr()), _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()),
+            _ld(hpsi_t), ei.ctypes.data_as(C.c_void_p), C.byref(ifail),
+            C.cast(stats, C.POINTER(C.c_int)), C.c_void_p(stream)))
+        keys = ("iters", "hpsi_products", "nvec", "mtxds", "fp64_from")
+        return ei, ifail.value, dict(zip(keys, list(stats)))
 
     def forces(self, psi_t, rat, avec, proj_atom, floc_q=None, floc_f=None, species=None,
                occ=None):

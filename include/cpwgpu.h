@@ -166,6 +166,62 @@ int cpwgpu_graph_destroy(cpwgpu_graph_t graph);
 int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hpsi_dev, long ldh,
                          int rotate, double *eig, double *res, void *stream);
 
+/* ---- iterative diagonalisation on the device --------------------------------- */
+
+/* Lowest neig eigenpairs of the plan's H, everything on the device; replaces
+ * Src/ditsp/ditsp_c16.f90 (with ditsp_c16_start / _loop / _ritz / ditsp_error)
+ * for a caller whose arrays live on the GPU:
+ *   prec        CPWGPU_PREC_FP64, or CPWGPU_PREC_FP32 = H in FP32-mixed until
+ *               the residuals reach ~1e-5, then FP64 H to the end (the result
+ *               is always to FP64 accuracy)
+ *   icmax       maximum number of iterations (each adds a block of relaxed
+ *               residuals and makes one Rayleigh-Ritz step)
+ *   iguess      1: psi_dev holds neig guess vectors; otherwise the start is
+ *               the eigenvectors of the small matrix (leading mtxds x mtxds
+ *               block of H), padded with zeros (ditsp_c16_start.f90:116,209-217)
+ *   epspsi      convergence: max_n ||H psi_n - e_n psi_n||^2 < epspsi^2 over
+ *               the neig bands (ditsp_c16.f90:375-377)
+ *   mtxds       size of the small matrix; <= 0: chosen from diag(H) by the
+ *               rule of Src/ALFA/size_mtxds.f90; >= mtxd: H is diagonalised
+ *               directly (no iterations)
+ *   hamsm_dev   device complex128 small matrix (ldhs, mtxds) or NULL: then
+ *               built on the device as the leading block of H
+ *   hdiag_dev   device diag(H) (mtxd) for the preconditioner, or NULL: the
+ *               exact diagonal (ekpg + V(G=0) + sum_a xnlkb_a |anlga_ia|^2)
+ *   psi_dev     (ldp, neig) in (guesses, iguess = 1) / out: orthonormal
+ *               eigenvectors;  hpsi_dev (ldh, neig) out: H psi in FP64
+ *   ei          host (neig) out: eigenvalues, ascending
+ *   ifail       host out: 0 converged, else 1 + int(-log10(max residual^2))
+ *               as ditsp_c16.f90:408-409 (may be NULL)
+ *   stats       host int[5] out (may be NULL): iterations, H|psi> products,
+ *               vectors iterated (neig + buffer), mtxds used, iteration at
+ *               which FP64 H took over (-1: FP64 throughout)
+ * Up to max(8, neig/8) buffer vectors beyond neig are iterated and not
+ * required to converge.  The small eigenproblems use rocSOLVER (zheevd,
+ * zpotrf) and rocBLAS (ztrsm) on the calling stream.  Returns when done;
+ * not converging within icmax is reported in ifail, not as an error. */
+int cpwgpu_plan_ditsp(cpwgpu_plan_t plan, int prec, int icmax, int iguess, double epspsi, int neig,
+                      int mtxds, const double *hamsm_dev, long ldhs, const double *hdiag_dev,
+                      double *psi_dev, long ldp, double *hpsi_dev, long ldh, double *ei, int *ifail,
+                      int *stats, void *stream);
+
+/* Drop-in for the reference's subroutine ditsp_c16 (Src/ditsp/ditsp_c16.f90:20-27):
+ * the same argument list, by reference, host arrays (column-major, leading
+ * dimensions mxddim for psi/hpsi/anlga and mxdsml for hamsm).  Precision as
+ * set by cpwgpu_set_precision.  psi/hpsi/ei are copied once per call; the H
+ * products of the whole iteration stay on the GPU.  The argument checks of
+ * ditsp_c16.f90:151-212 return CPWGPU_ERR_ARG / _MTXD / _NG (the reference
+ * stops); neig < 1 prints the reference's warning and returns.  When mtxds >=
+ * mtxd hpsi is also written (the reference leaves it unset there).  lnewanl
+ * is never written; neig is not changed. */
+int cpwgpu_ditsp_c16(const int *ipr, int *ifail, const int *icmax, const int *iguess, const double *epspsi,
+                     const int *lnewanl, int *neig, const int *mtxd, const int *mtxds, double *psi,
+                     double *hpsi, double *ei, const double *ekpg, const int *isort, const double *vscr,
+                     const int *kmscr, const int *ng, const int *kgv, const double *anlga, const double *xnlkb,
+                     const int *nanl, const double *hamsm, const double *hdiag, const int *mxddim,
+                     const int *mxdsml, const int *mxdbnd, const int *mxdgve, const int *mxdscr,
+                     const int *mxdanl);
+
 /* ---- atomic forces ----------------------------------------------------------- */
 
 /* Forces on the atoms from E = sum_n occ_n <psi_n| V_loc + V_NL |psi_n> at

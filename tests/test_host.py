@@ -72,13 +72,53 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {n}\b", out), n
 
 
-def test_fortran_shim_exports_hk_psi_c16():
+def test_fortran_shim_exports_hk_psi_c16_and_ditsp_c16():
     from cpw2000_amd import hk_psi
 
     assert os.path.exists(hk_psi.FLIB_PATH)
     out = subprocess.run(["nm", "-D", "--defined-only", hk_psi.FLIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     assert re.search(r"\bT hk_psi_c16_\b", out)
+    assert re.search(r"\bT ditsp_c16_\b", out)
+
+
+def _ditsp_args(d, neig, **over):
+    """Arguments of ditsp_c16 for a config dict (host arrays, Fortran layout)."""
+    mtxd, mxddim = d["mtxd"], d["mxddim"]
+    a = dict(ipr=0, icmax=50, iguess=0, epspsi=1e-6, lnewanl=True, neig=neig, mtxd=mtxd,
+             mtxds=3 * neig + 20, mxdsml=3 * neig + 20, mxdbnd=neig)
+    a.update(over)
+    psi = np.zeros((mxddim, a["mxdbnd"]), np.complex128, order="F")
+    out = dict(
+        a, psi=psi, hpsi=np.zeros_like(psi, order="F"), ei=np.zeros(a["mxdbnd"]),
+        ekpg=np.asfortranarray(d["ekpg"]), isort=np.asfortranarray(d["isort"], np.int32),
+        vscr=np.asfortranarray(d["vscr"]), kmscr=np.asfortranarray(d["kmscr"], np.int32),
+        ng=d["ng"], kgv=np.asfortranarray(d["kgv"], np.int32),
+        anlga=np.asfortranarray(d["anlga"], np.complex128), xnlkb=np.asfortranarray(d["xnlkb"]),
+        nanl=d["nanl"], hamsm=np.zeros((a["mxdsml"], a["mxdsml"]), np.complex128, order="F"),
+        hdiag=np.asfortranarray(d["ekpg"].copy()), mxddim=mxddim, mxdgve=d["mxdgve"],
+        mxdscr=d["mxdscr"], mxdanl=max(d["mxdanl"], 1))
+    out.update(over)
+    return out
+
+
+def test_ditsp_argument_checks_of_the_reference():
+    """cpwgpu_ditsp_c16 reports the stop conditions of Src/ditsp/ditsp_c16.f90:151-212
+    before any device work; neig < 1 returns with ifail = 0."""
+    from cpw2000_amd import hk_psi
+
+    d = synth.make_config(0, nbands=2)
+    assert hk_psi.ditsp_c16(**_ditsp_args(d, 0, mtxds=8, mxdsml=8, mxdbnd=1)) == 0
+    for over, code in ((dict(mtxds=4), hk_psi.ERR_ARG),                 # neig > mtxds
+                       (dict(mxdbnd=4), hk_psi.ERR_ARG),                # neig > mxdbnd
+                       (dict(mxdsml=20, mtxds=20), hk_psi.ERR_ARG),     # 3 neig > mxdsml
+                       (dict(mtxds=60), hk_psi.ERR_ARG),                # mtxds > mxdsml
+                       (dict(mtxd=d["mxddim"] + 1), hk_psi.ERR_MTXD),   # mtxd > mxddim
+                       (dict(mxdgve=d["ng"] - 1), hk_psi.ERR_NG)):      # ng > mxdgve
+        args = _ditsp_args(d, 8, **over)
+        with pytest.raises(hk_psi.CpwGpuError) as e:
+            hk_psi.ditsp_c16(**args)
+        assert e.value.code == code, over
 
 
 def test_argument_errors_before_any_device_work():
