@@ -1290,12 +1290,12 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
 
 // ---- atomic forces ------------------------------------------------------------
 struct ForceWs {
-    DevBuf t2, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p;
+    DevBuf t2, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p, rhor;
 } g_fw;
 
 template <int N>
 int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *occ, int nb, int bpg,
-                   int ngrp, int Q1, int Q2, double2 *t2, hipStream_t s) {
+                   int ngrp, int Q1, int Q2, double2 *t2, hipStream_t s, double *rhor = nullptr) {
     const FftGeom &g = p->g;
     constexpr int RB = plane_rows_fast<N>();
     const size_t dyn = 16 * ((size_t)N * (g.nx | 1) + (size_t)RB * row_stride<N>()) + 8 * (size_t)N * N;
@@ -1309,7 +1309,41 @@ int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *o
     }
     hipLaunchKernelGGL((k_density_fast<N, N>), dim3(g.n[2] * ngrp), dim3(FT), dyn, s, colbuf,
                        (const int *)p->col_uy.p, occ, (const double2 *)p->tw[0].p,
-                       (const double2 *)p->tw[1].p, g, nb, bpg, Q1, Q2, t2);
+                       (const double2 *)p->tw[1].p, g, nb, bpg, Q1, Q2, t2, rhor);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// Electron density on the mesh, the band loop of Src/omp/charge_by_fft.f90:170-197:
+// rho(r) = sum_n occ_n |psi_n(r)|^2, psi_n(r) the unnormalised G -> r transform.
+// The inverse transforms are those of H|psi> (k_col_bwd, then y and x per plane).
+int plan_density(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const double *occ_h, double *rho,
+                 hipStream_t s) {
+    const FftGeom &g = p->g;
+    if (!fast_sizes(g)) return fail(CPWGPU_ERR_SIZE, "the density needs a mesh with a two-pass split");
+    int rc;
+    const int bpg = 8, ngrp = (neig + bpg - 1) / bpg;
+    const long nmesh = (long)g.n[0] * g.n[1] * g.n[2];
+    if ((rc = p->colbuf.reserve(16 * (size_t)g.n[2] * g.ncolp * neig)) ||
+        (rc = g_fw.rhor.reserve(8 * (size_t)ngrp * nmesh)))
+        return rc;
+    const double *occ = nullptr;
+    if (occ_h) {
+        if ((rc = upload(g_fw.occ, occ_h, (size_t)neig))) return rc;
+        occ = (const double *)g_fw.occ.p;
+        HIPCHK(hipStreamSynchronize(g_stream));  // the upload went through the library stream
+    }
+    const LocalArgs la{psi, ldp, nullptr, 0, neig, 0};
+    if ((rc = local_stage<double>(p, la, (cplx<double> *)p->colbuf.p, (const double *)p->v64.p, ST_COL_BWD, s)))
+        return rc;
+    rc = CPWGPU_ERR_SIZE;
+#define CPW_DEN(N) if (g.n[0] == N) rc = launch_density<N>(p, (const cplx<double> *)p->colbuf.p, occ, neig, bpg, ngrp, 0, 0, nullptr, s, (double *)g_fw.rhor.p);
+    CPW_FAST_SIZES(CPW_DEN)
+#undef CPW_DEN
+    if (rc) return rc;
+    const long tot = (long)g.id * g.n[1] * g.n[2];
+    hipLaunchKernelGGL(k_rho_r_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       (const double *)g_fw.rhor.p, ngrp, g.n[0], g.n[1], g.n[2], g.id, rho);
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
 }
@@ -1604,7 +1638,8 @@ int cpwgpu_finalize(void) {
     }
     for (DevBuf *b : {&g_dt.xa, &g_dt.hxa, &g_dt.xb, &g_dt.hxb, &g_dt.pb, &g_dt.cpk, &g_dt.hd, &g_dt.m,
                       &g_dt.g, &g_dt.g0,
-                      &g_dt.pm, &g_dt.w, &g_dt.e, &g_dt.info, &g_dt.res, &g_dt.act, &g_dt.d, &g_dt.mones})
+                      &g_dt.pm, &g_dt.w, &g_dt.e, &g_dt.info, &g_dt.res, &g_dt.act, &g_dt.d, &g_dt.mones,
+                      &g_fw.rhor})
         b->release();
     if (g_aux) {
         HIPCHK(hipStreamSynchronize(g_aux));
@@ -1726,6 +1761,16 @@ int cpwgpu_rayleigh_ritz(int mtxd, int nb, double *psi_dev, long ldp, double *hp
     if (st) return st;
     return rayleigh_ritz(mtxd, nb, (double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, rotate, eig, res,
                          (hipStream_t)stream);
+}
+
+int cpwgpu_plan_density(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, const double *occ,
+                        double *rho_dev, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || neig < 1 || !psi_dev || !rho_dev || ldp < plan->mtxd)
+        return fail(CPWGPU_ERR_ARG, "bad density arguments");
+    int st = ensure_stream();
+    if (st) return st;
+    return plan_density(plan, neig, (const double2 *)psi_dev, ldp, occ, rho_dev, (hipStream_t)stream);
 }
 
 int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,

@@ -34,7 +34,8 @@ __global__ __launch_bounds__(FT) void k_density_fast(const cplx<double> *__restr
                                                      const double2 *__restrict__ tw1g,
                                                      const double2 *__restrict__ tw2g, FftGeom g,
                                                      int nb, int bpg, int Q1, int Q2,
-                                                     double2 *__restrict__ t2) {
+                                                     double2 *__restrict__ t2,
+                                                     double *__restrict__ rhor = nullptr) {
     typedef double R;
     constexpr int A1 = split_a(N1), B1 = N1 / A1;
     constexpr int RB = FT / B1;
@@ -95,6 +96,13 @@ __global__ __launch_bounds__(FT) void k_density_fast(const cplx<double> *__restr
             __syncthreads();
         }
     }
+    if (rhor) {
+        // rho(x, y) of this plane and band group, for a fixed-order sum over groups
+        __syncthreads();
+        double *o = rhor + ((long)grp * n3 + z) * (N2 * N1);
+        for (int q = threadIdx.x; q < N2 * N1; q += FT) o[q] = acc[q];
+        return;
+    }
     // partial transform along x into sub: T1[y][q1 + Q1]
     const int W1 = 2 * Q1 + 1, W2 = 2 * Q2 + 1;
     for (int idx = threadIdx.x; idx < N2 * W1; idx += FT) {
@@ -126,6 +134,24 @@ __global__ __launch_bounds__(FT) void k_density_fast(const cplx<double> *__restr
         }
         out[idx] = make_double2(sr, si);
     }
+}
+
+// rho[(z n2 + y) id + x] = sum over band groups of part[grp][z][y][x] in
+// group order (x < n1; the padding x >= n1 of the id-wide layout is zero)
+__global__ __launch_bounds__(256) void k_rho_r_reduce(const double *__restrict__ part, int ngrp, int n1,
+                                                      int n2, int n3, int id, double *__restrict__ rho) {
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    const long tot = (long)id * n2 * n3;
+    if (q >= tot) return;
+    const int x = (int)(q % id);
+    const long yz = q / id;
+    double s = 0.0;
+    if (x < n1) {
+        const long nmesh = (long)n1 * n2 * n3;
+        const long src = yz * n1 + x;
+        for (int gq = 0; gq < ngrp; gq++) s += part[gq * nmesh + src];
+    }
+    rho[q] = s;
 }
 
 // rho(q) for each listed q: (1/N) sum_grp sum_z T2 exp(-2 pi i q3 z / N3)

@@ -48,7 +48,7 @@ EXPORTS = (
     "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz", "cpwgpu_plan_forces",
     "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
     "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune",
-    "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16",
+    "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
@@ -105,6 +105,7 @@ def lib():
         L.cpwgpu_plan_get_variant.argtypes = [vp, C.c_int, C.c_int, ip]
         L.cpwgpu_plan_tune.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
+        L.cpwgpu_plan_density.argtypes = [vp, C.c_int, dp, C.c_long, dp, dp, vp]
         L.cpwgpu_plan_ditsp.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                         dp, C.c_long, dp, dp, C.c_long, dp, C.c_long, dp, ip, ip, vp]
         _lib = L
@@ -336,6 +337,22 @@ class Plan:
                                        _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
                                        int(prec), C.c_void_p(stream)))
         return hpsi_t
+
+    def density(self, psi_t, occ=None, stream=None):
+        """rho(r) = sum_n occ_n |psi_n(r)|^2 on the mesh (cpwgpu_plan_density):
+        a float64 device tensor of id*n2*n3 values in the vscr layout."""
+        import torch
+
+        info = self.info()
+        rho = torch.empty(info["id"] * info["n2"] * info["n3"], dtype=torch.float64,
+                          device=psi_t.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        occ_a = None if occ is None else np.ascontiguousarray(occ, np.float64)
+        _check(lib().cpwgpu_plan_density(
+            self._h, int(psi_t.shape[0]), C.c_void_p(psi_t.data_ptr()), _ld(psi_t),
+            None if occ_a is None else _p(occ_a), C.c_void_p(rho.data_ptr()), C.c_void_p(stream)))
+        return rho
 
     def ditsp(self, psi_t, hpsi_t, prec=PREC_FP64, icmax=100, iguess=0, epspsi=1e-6, mtxds=0,
               hamsm_t=None, hdiag_t=None, stream=None):

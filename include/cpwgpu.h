@@ -222,6 +222,18 @@ int cpwgpu_ditsp_c16(const int *ipr, int *ifail, const int *icmax, const int *ig
                      const int *mxdsml, const int *mxdbnd, const int *mxdgve, const int *mxdscr,
                      const int *mxdanl);
 
+/* ---- electron density ---------------------------------------------------------- */
+
+/* rho(r) = sum_n occ_n |psi_n(r)|^2 on the plan's FFT mesh, psi_n(r) =
+ * sum_G psi_n(G) exp(+i G.r) (the unnormalised transform of cfft_wf_c16 mode
+ * -1), FP64: the band loop of Src/omp/charge_by_fft.f90:170-197 (array
+ * rhomsh).  psi_dev (ldp, neig) device complex128; occ host (neig) or NULL
+ * (= 1); rho_dev device double in the vscr layout rho[(k3*n2 + k2)*id + k1]
+ * (id*n2*n3 values, zero for k1 >= n1).  The mesh must be one of the
+ * specialised sizes.  Enqueued on stream. */
+int cpwgpu_plan_density(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, const double *occ,
+                        double *rho_dev, void *stream);
+
 /* ---- atomic forces ----------------------------------------------------------- */
 
 /* Forces on the atoms from E = sum_n occ_n <psi_n| V_loc + V_NL |psi_n> at
