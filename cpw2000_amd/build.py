@@ -1,7 +1,7 @@
 """Build recipe for the native libraries of cpw2000_amd (in-tree).
 
   libcpwgpu.so          HIP kernels + C ABI (include/cpwgpu.h), gfx950
-  libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16 and ditsp_c16 over the C ABI (flang)
+  libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16, hk_psi_spin_c16 and ditsp_c16 over the C ABI (flang)
 
 Run ``python -m cpw2000_amd.build``.
 """
@@ -47,7 +47,8 @@ def build_hip():
 
 
 def build_fortran():
-    srcs = [os.path.join(PKG, "fortran", f) for f in ("hk_psi_c16.f90", "ditsp_c16.f90")]
+    srcs = [os.path.join(PKG, "fortran", f)
+            for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90")]
     tmp = FLIB + ".tmp"
     objdir = os.path.join(PKG, "fortran")
     _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, *srcs,

@@ -27,6 +27,7 @@
 #include "subspace.hpp"
 #include "forces.hpp"
 #include "ditsp.hpp"
+#include "spin.hpp"
 
 using namespace cpwgpu;
 
@@ -195,11 +196,22 @@ struct cpwgpu_plan_s {
     // workspaces
     DevBuf colbuf, part, dhd;
     DevBuf gw;                // g_j of each plane wave as doubles [3][mtxd] (force projections)
+    // spinor path (hk_psi_spin_c16): potential components in the layout of
+    // k_spin_potential, projectors on the 2*mtxd interleaved rows, split bands
+    int nsp = 0;
+    DevBuf vsp64, vsp32;
+    DevBuf sanl64, sanl32, sxnl;
+    long slda = 0, slda32 = 0;
+    int snanl = 0;
+    bool sanl32_valid = false;
+    DevBuf spsi, shpsi;
     ~cpwgpu_plan_s() {
         col_ptr.release(); col_uy.release(); ent_i.release(); ent_cz.release(); ekpg.release();
         for (auto &t : tw) t.release();
         v64.release(); v32.release(); anl64.release(); anl32.release(); xnl.release();
         colbuf.release(); part.release(); dhd.release(); gw.release();
+        vsp64.release(); vsp32.release(); sanl64.release(); sanl32.release(); sxnl.release();
+        spsi.release(); shpsi.release();
     }
 };
 
@@ -544,30 +556,54 @@ int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
 
 // Kleinman-Bylander term: hpsi = beta*hpsi + anlga Diag(xnlkb) anlga^H psi
 // (Src/cpw_lib/hk_psi_nl_c16.f90:78-96) on stream s.
+// One set of projectors: the plan's (mtxd rows) or the spinor ones (2*mtxd
+// interleaved rows, Src/hk_psi/reference/hk_psi_spin_c16.f90:302-303).
+struct ProjView {
+    DevBuf *anl64, *anl32;
+    bool *anl32_valid;
+    long lda;
+    long *lda32;
+    int nanl;
+    const double *xnl;
+    int rows;
+};
+
+ProjView scalar_proj(cpwgpu_plan_s *p) {
+    return {&p->anl64, &p->anl32, &p->anl32_valid, p->lda, &p->lda32, p->nanl, (const double *)p->xnl.p,
+            p->g.mtxd};
+}
+
+ProjView spin_proj(cpwgpu_plan_s *p) {
+    return {&p->sanl64, &p->sanl32, &p->sanl32_valid, p->slda, &p->slda32, p->snanl,
+            (const double *)p->sxnl.p, 2 * p->g.mtxd};
+}
+
+// hpsi = beta*hpsi + sum_a beta_a xnlkb_a <beta_a|psi> (Src/cpw_lib/hk_psi_nl_c16.f90:82-96),
+// plus ekpg*psi when kin (the kinetic term, only for the plan's own rows)
 template <typename R>
-int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-             int beta, hipStream_t s) {
-    const FftGeom &g = p->g;
+int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi, long ldp, double *hpsi,
+             long ldh, int beta, bool kin, hipStream_t s) {
     const size_t cs = sizeof(cplx<R>);
-    const int nanl = p->nanl;
+    const int nanl = pv_.nanl;
+    const int rows = pv_.rows;
     int st;
     const cplx<R> *anl;
     if (sizeof(R) == 8) {
-        anl = (const cplx<R> *)p->anl64.p;
+        anl = (const cplx<R> *)pv_.anl64->p;
     } else {
-        if (!p->anl32_valid) {
-            p->lda32 = (p->lda + 1) & ~1L;
-            if ((st = p->anl32.reserve(sizeof(cplx<float>) * (size_t)p->lda32 * nanl))) return st;
-            const long tot = (long)g.mtxd * nanl;
+        if (!*pv_.anl32_valid) {
+            *pv_.lda32 = (pv_.lda + 1) & ~1L;
+            if ((st = pv_.anl32->reserve(sizeof(cplx<float>) * (size_t)*pv_.lda32 * nanl))) return st;
+            const long tot = (long)rows * nanl;
             hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                               (const double2 *)p->anl64.p, p->lda, (cplx<float> *)p->anl32.p,
-                               p->lda32, g.mtxd, nanl);
+                               (const double2 *)pv_.anl64->p, pv_.lda, (cplx<float> *)pv_.anl32->p,
+                               *pv_.lda32, rows, nanl);
             HIPCHK(hipGetLastError());
-            p->anl32_valid = true;
+            *pv_.anl32_valid = true;
         }
-        anl = (const cplx<R> *)p->anl32.p;
+        anl = (const cplx<R> *)pv_.anl32->p;
     }
-    const long lda = sizeof(R) == 8 ? p->lda : p->lda32;
+    const long lda = sizeof(R) == 8 ? pv_.lda : *pv_.lda32;
     const int ix = sizeof(R) == 4 ? 1 : 0;
     const int pv = p->proj_var[ix] >= 1 && p->proj_var[ix] <= NPROJ_DIRECT ? p->proj_var[ix] : 0;
     // block tile and K granule of the chosen projection kernel
@@ -577,11 +613,11 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
 #undef CPW_PD
     const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
     // K slices are whole LDS stages of k_proj / whole chunks of k_proj_direct
-    const int ksteps = (g.mtxd + BK - 1) / BK;
+    const int ksteps = (rows + BK - 1) / BK;
     const int target = p->splitk[ix] > 0 ? p->splitk[ix] : 256;
     int nsl = std::max(1, std::min(ksteps, (target + tm * tn - 1) / (tm * tn)));
     const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
-    nsl = (g.mtxd + kslice - 1) / kslice;
+    nsl = (rows + kslice - 1) / kslice;
     if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
     if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
     {
@@ -590,21 +626,21 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
         const double2 *ps = (const double2 *)psi;
         cplx<R> *pt = (cplx<R> *)p->part.p;
         if (pv == 1)
-            hipLaunchKernelGGL((k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+            hipLaunchKernelGGL((k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else if (pv == 2)
-            hipLaunchKernelGGL((k_proj_direct<R, 2>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+            hipLaunchKernelGGL((k_proj_direct<R, 2>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else if (pv == 3)
-            hipLaunchKernelGGL((k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+            hipLaunchKernelGGL((k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else if (pv == 4)
-            hipLaunchKernelGGL((k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, g.mtxd, nanl, neig, kslice);
+            hipLaunchKernelGGL((k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else
             hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, g.mtxd, nanl, neig,
+                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, rows, nanl, neig,
                                kslice, nullptr);
     }
     {
         ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
-        launch_proj_reduce<R>((const cplx<R> *)p->part.p, (const double *)p->xnl.p,
+        launch_proj_reduce<R>((const cplx<R> *)p->part.p, pv_.xnl,
                               (cplx<R> *)p->dhd.p, nanl, neig, nsl, s);
     }
     {
@@ -613,20 +649,20 @@ int nonlocal(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hp
         const cplx<R> *dh = (const cplx<R> *)p->dhd.p;
         double2 *hp = (double2 *)hpsi;
         // the kinetic term ekpg * psi is added here (coalesced), not in the gather
-        const double2 *kp = (const double2 *)psi;
-        const double *ek = (const double *)p->ekpg.p;
+        const double2 *kp = kin ? (const double2 *)psi : nullptr;
+        const double *ek = kin ? (const double *)p->ekpg.p : nullptr;
 #define CPW_BD(V)                                                                                  \
     if (bv == V)                                                                                   \
         hipLaunchKernelGGL((k_back_direct<R, V>),                                                  \
-                           dim3((g.mtxd + back_bi<R, V>() - 1) / back_bi<R, V>(),                  \
+                           dim3((rows + back_bi<R, V>() - 1) / back_bi<R, V>(),                  \
                                 (neig + back_bn<R, V>() - 1) / back_bn<R, V>()),                   \
-                           dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta, kp,   \
+                           dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta, kp,   \
                            ldp, ek);
         CPW_BD(1) CPW_BD(2) CPW_BD(3)
 #undef CPW_BD
         if (bv == 0)
-            hipLaunchKernelGGL(k_back<R>, dim3((g.mtxd + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
-                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, g.mtxd, nanl, neig, beta,
+            hipLaunchKernelGGL(k_back<R>, dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta,
                                kp, ldp, ek);
     }
     HIPCHK(hipGetLastError());
@@ -690,7 +726,7 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
     int st;
     if (nl && !g_overlap) {
         // serial: non-local term first (writes hpsi), then the local term adds
-        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, s))) return st;
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, true, s))) return st;
         return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s);
     }
     if (nl) {
@@ -698,7 +734,7 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
         if ((st = ensure_aux())) return st;
         HIPCHK(hipEventRecord(g_fork, s));
         HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
-        if ((st = nonlocal<R>(p, neig, psi, ldp, hpsi, ldh, 0, g_aux))) return st;
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, true, g_aux))) return st;
         HIPCHK(hipEventRecord(g_join, g_aux));
         return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s);
     }
@@ -749,12 +785,163 @@ int set_projectors_host(cpwgpu_plan_s *p, const double *anlga, const double *xnl
     return CPWGPU_OK;
 }
 
+// ---- two-component H|psi> (Src/hk_psi/reference/hk_psi_spin_c16.f90) -------------
+DevBuf g_spin_tmp;  // staging of vscr_sp
+
+int set_potential_spin(cpwgpu_plan_s *p, const double *vscr_sp, int mxdscr, int nsp) {
+    if (nsp != 1 && nsp != 2 && nsp != 4)
+        return fail(CPWGPU_ERR_MODE, "nsp = %d is an incorrect value, should be 1,2,4", nsp);
+    if (p->mwrk > mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", p->mwrk, mxdscr);
+    if (p->ntot > (size_t)mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mesh id*n2*n3 = %zu is greater than mxdscr = %d", p->ntot, mxdscr);
+    int st = ensure_stream();
+    if (st) return st;
+    const long ntot = (long)p->ntot;
+    const size_t ncomp = nsp == 1 ? 1 : nsp == 2 ? 2 : 4;
+    if ((st = upload(g_spin_tmp, vscr_sp, (size_t)mxdscr * nsp)) || (st = p->vsp64.reserve(8 * ncomp * ntot)) ||
+        (st = p->vsp32.reserve(4 * ncomp * ntot)))
+        return st;
+    const dim3 grid((unsigned)((ntot + SP_THREADS - 1) / SP_THREADS));
+    hipLaunchKernelGGL(k_spin_potential<double>, grid, dim3(SP_THREADS), 0, g_stream, (const double *)g_spin_tmp.p,
+                       (long)mxdscr, nsp, ntot, (double *)p->vsp64.p);
+    hipLaunchKernelGGL(k_spin_potential<float>, grid, dim3(SP_THREADS), 0, g_stream, (const double *)g_spin_tmp.p,
+                       (long)mxdscr, nsp, ntot, (float *)p->vsp32.p);
+    HIPCHK(hipGetLastError());
+    p->nsp = nsp;
+    return CPWGPU_OK;
+}
+
+// anlsp(2*mxddim, nanlsp) complex128 host -> 2*mtxd rows on the device
+int set_projectors_spin_host(cpwgpu_plan_s *p, const double *anlsp, const double *xnlkbsp, int nanlsp,
+                             int mxddim) {
+    if (nanlsp < 0) return fail(CPWGPU_ERR_ARG, "nanlsp = %d", nanlsp);
+    int st = ensure_stream();
+    if (st) return st;
+    p->snanl = nanlsp;
+    p->sanl32_valid = false;
+    if (nanlsp == 0) return CPWGPU_OK;
+    if (!anlsp || !xnlkbsp) return fail(CPWGPU_ERR_ARG, "null projector array");
+    p->slda = std::max(1, 2 * p->mtxd);
+    if ((st = p->sanl64.reserve(16 * (size_t)p->slda * nanlsp))) return st;
+    if (p->mtxd > 0)
+        HIPCHK(hipMemcpy2DAsync(p->sanl64.p, 16 * (size_t)p->slda, anlsp, 16 * (size_t)(2 * (size_t)mxddim),
+                                16 * (size_t)(2 * p->mtxd), nanlsp, hipMemcpyHostToDevice, g_stream));
+    return upload(p->sxnl, xnlkbsp, nanlsp);
+}
+
+template <typename R, int N>
+int launch_plane_spin(cpwgpu_plan_s *p, cplx<R> *colbuf, const R *vr, long vstride, int npair, hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t dyn = sizeof(cplx<R>) * (2 * (size_t)N * plane_sub_stride(g.nx) +
+                                          (size_t)plane_rows_fast<N>() * row_stride<N>());
+    const size_t stat = sizeof(cplx<R>) * 2 * N;
+    if (dyn + stat > 160 * 1024)
+        return fail(CPWGPU_ERR_SIZE, "two-component plane of %d x %d does not fit in LDS", N, N);
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_plane_spin<R, N, N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(160 * 1024 - stat)));
+        attr = true;
+    }
+    ProfScope ps_(CPWGPU_K_PLANE, s);
+    hipLaunchKernelGGL((k_plane_spin<R, N, N>), dim3(g.n[2] * npair), dim3(FT), dyn, s, colbuf,
+                       (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p,
+                       (const double2 *)p->tw[1].p, g, npair);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// H psi_sp for neig two-component bands (rows 2i / 2i+1 = up / down of plane
+// wave i): the components are split into scalar bands, the local part and
+// the kinetic term go through the scalar column kernels (the potential per
+// component, or the coupled plane kernel for nsp = 4), the non-local GEMMs run
+// on the interleaved rows beside them, and the two are merged.
+template <typename R>
+int apply_spin(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, double2 *hpsi, long ldh, hipStream_t s) {
+    const FftGeom &g = p->g;
+    const int mtxd = g.mtxd;
+    if (!p->nsp) return fail(CPWGPU_ERR_ARG, "two-component potential not set");
+    if (p->nsp == 4 && g.ncol > 0 && !fast_sizes(g))
+        return fail(CPWGPU_ERR_SIZE, "the non-collinear potential needs one of the specialised meshes");
+    if (neig == 0 || mtxd == 0) return CPWGPU_OK;
+    int st;
+    const size_t colb = 16 * (size_t)mtxd;
+    if ((st = p->spsi.reserve(colb * 2 * neig)) || (st = p->shpsi.reserve(colb * 2 * neig))) return st;
+    double2 *sp = (double2 *)p->spsi.p, *sh = (double2 *)p->shpsi.p;
+    const dim3 eg((unsigned)((mtxd + SP_THREADS - 1) / SP_THREADS), (unsigned)neig);
+    hipLaunchKernelGGL(k_spin_split, eg, dim3(SP_THREADS), 0, s, psi, ldp, mtxd, neig, sp);
+    HIPCHK(hipGetLastError());
+    const bool nl = p->snanl > 0;
+    if (nl) {
+        if ((st = ensure_aux())) return st;
+        HIPCHK(hipEventRecord(g_fork, s));
+        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+        if ((st = nonlocal<R>(p, spin_proj(p), neig, (const double *)psi, ldp, (double *)hpsi, ldh, 0, false,
+                              g_aux)))
+            return st;
+        HIPCHK(hipEventRecord(g_join, g_aux));
+    }
+    const R *vr = sizeof(R) == 8 ? (const R *)p->vsp64.p : (const R *)p->vsp32.p;
+    const long ntot = (long)p->ntot;
+    const size_t per_band = sizeof(cplx<R>) * (size_t)g.n[2] * g.ncolp;
+    // scalar bands per pass: all of them if the column buffer stays under 4 GiB
+    const int nbc = (int)std::max<size_t>(2, std::min<size_t>(2 * (size_t)neig, (4ull << 30) / per_band)) & ~1;
+    if ((st = p->colbuf.reserve(per_band * nbc))) return st;
+    cplx<R> *colbuf = (cplx<R> *)p->colbuf.p;
+    auto columns = [&](int stage, int b0, int nb, cplx<R> *cbuf) -> int {
+        const LocalArgs la{sp + (long)b0 * mtxd, mtxd, sh + (long)b0 * mtxd, mtxd, nb, COLF_KIN};
+        return local_stage<R>(p, la, cbuf, vr, stage, s);
+    };
+    if (g.ncol > 0 && p->nsp != 4) {
+        // potential diagonal in spin: scalar passes, up bands [0, neig) and down bands [neig, 2 neig)
+        for (int b0 = 0; b0 < 2 * neig; b0 += nbc) {
+            const int nb = std::min(nbc, 2 * neig - b0);
+            if ((st = columns(ST_COL_BWD, b0, nb, colbuf))) return st;
+            // bands of one component share a potential: split the plane pass at neig
+            for (int c0 = b0; c0 < b0 + nb;) {
+                const int comp = c0 < neig ? 0 : 1;
+                const int c1 = comp == 0 ? std::min(b0 + nb, neig) : b0 + nb;
+                const LocalArgs la{nullptr, 0, nullptr, 0, c1 - c0, COLF_KIN};
+                const R *vc = vr + (p->nsp == 2 ? comp * ntot : 0);
+                if ((st = local_stage<R>(p, la, colbuf + (size_t)(c0 - b0) * g.n[2] * g.ncolp, vc, ST_PLANE, s)))
+                    return st;
+                c0 = c1;
+            }
+            if ((st = columns(ST_COL_FWD, b0, nb, colbuf))) return st;
+        }
+    } else if (g.ncol > 0) {
+        // coupled: pairs (up n, down n) in one plane kernel
+        const int npc = nbc / 2;
+        for (int n0 = 0; n0 < neig; n0 += npc) {
+            const int np_ = std::min(npc, neig - n0);
+            cplx<R> *cb_dn = colbuf + (size_t)np_ * g.n[2] * g.ncolp;
+            if ((st = columns(ST_COL_BWD, n0, np_, colbuf)) || (st = columns(ST_COL_BWD, neig + n0, np_, cb_dn)))
+                return st;
+            st = CPWGPU_ERR_SIZE;
+#define CPW_SPL(N) if (g.n[0] == N) st = launch_plane_spin<R, N>(p, colbuf, vr, ntot, np_, s);
+            CPW_FAST_SIZES(CPW_SPL)
+#undef CPW_SPL
+            if (st) return st;
+            if ((st = columns(ST_COL_FWD, n0, np_, colbuf)) || (st = columns(ST_COL_FWD, neig + n0, np_, cb_dn)))
+                return st;
+        }
+    } else {
+        // no z column (mtxd = 0 handled above): nothing local
+    }
+    if (nl) HIPCHK(hipStreamWaitEvent(s, g_join, 0));
+    hipLaunchKernelGGL(k_spin_merge, eg, dim3(SP_THREADS), 0, s, (const double2 *)sh, mtxd, neig, hpsi, ldh,
+                       nl ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
 // ---- drop-in state --------------------------------------------------------
 struct DropIn {
     cpwgpu_plan_s *plan = nullptr;
     int key_ng = -1, key_mtxd = -1;
     int key_kmscr[7] = {0};
-    DevBuf psi, hpsi, hs, hd;
+    DevBuf psi, hpsi, hs, hd, c8;
 } g_drop;
 
 int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
@@ -1631,6 +1818,8 @@ int cpwgpu_finalize(void) {
     g_drop.hpsi.release();
     g_drop.hs.release();
     g_drop.hd.release();
+    g_drop.c8.release();
+    g_spin_tmp.release();
     if (g_blas) {
         if (g_stream) HIPCHK(hipStreamSynchronize(g_stream));
         rocblas_destroy_handle(g_blas);
@@ -1997,6 +2186,147 @@ int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, doubl
     if (st) return st;
     HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)*mxddim, g_drop.hpsi.p, colb, colb, *neig,
                             hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_potential_spin(cpwgpu_plan_t plan, const double *vscr_sp, int mxdscr, int nsp) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || !vscr_sp) return fail(CPWGPU_ERR_ARG, "null argument");
+    int st = set_potential_spin(plan, vscr_sp, mxdscr, nsp);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_set_projectors_spin(cpwgpu_plan_t plan, const double *anlsp, const double *xnlkbsp, int nanlsp,
+                                    int mxddim) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (mxddim < plan->mtxd) return fail(CPWGPU_ERR_MTXD, "mtxd = %d, mxddim = %d", plan->mtxd, mxddim);
+    int st = set_projectors_spin_host(plan, anlsp, xnlkbsp, nanlsp, mxddim);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_apply_spin(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                           long ldh, int prec, void *stream) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
+    if (neig == 0) return CPWGPU_OK;
+    if (!psi_dev || !hpsi_dev) return fail(CPWGPU_ERR_ARG, "null psi/hpsi");
+    if (ldp < 2L * plan->mtxd || ldh < 2L * plan->mtxd) return fail(CPWGPU_ERR_ARG, "leading dimension < 2*mtxd");
+    int st = ensure_stream();
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == CPWGPU_PREC_FP64)
+        return apply_spin<double>(plan, neig, (const double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, s);
+    if (prec == CPWGPU_PREC_FP32)
+        return apply_spin<float>(plan, neig, (const double2 *)psi_dev, ldp, (double2 *)hpsi_dev, ldh, s);
+    return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+}
+
+int cpwgpu_hk_psi_spin_c16(const int *mtxd, const int *neig, const double *psi_sp, double *hpsi_sp,
+                           const int *lnewanl, const int *ng, const int *kgv, const double *ekpg, const int *isort,
+                           const double *vscr_sp, const int *kmscr, const int *nsp, const double *anlsp,
+                           const double *xnlkbsp, const int *nanlsp, const int *mxddim, const int *mxdbnd,
+                           const int *mxdasp, const int *mxdgve, const int *mxdscr, const int *mxdnsp) {
+    (void)lnewanl;
+    (void)mxdbnd;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!mtxd || !neig || !ng || !kmscr || !nsp || !nanlsp || !mxddim || !mxdasp || !mxdgve || !mxdscr || !mxdnsp)
+        return fail(CPWGPU_ERR_ARG, "null scalar argument");
+    // the checks of Src/hk_psi/reference/hk_psi_spin_c16.f90:106-141, in the same order
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", *ng, *mxdgve);
+    if (*nsp > *mxdnsp) return fail(CPWGPU_ERR_ARG, "nsp = %d is greater than mxdnsp = %d", *nsp, *mxdnsp);
+    if (*nsp != 1 && *nsp != 2 && *nsp != 4)
+        return fail(CPWGPU_ERR_MODE, "nsp = %d is an incorrect value, should be 1,2,4", *nsp);
+    int ns[3], mfft, mwrk;
+    size_fft(kmscr, ns, &mfft, &mwrk);
+    if (mwrk > *mxdscr) return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", mwrk, *mxdscr);
+    if (*nanlsp > *mxdasp) return fail(CPWGPU_ERR_ARG, "nanlsp = %d > mxdasp = %d", *nanlsp, *mxdasp);
+    if (*neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", *neig);
+    cpwgpu_plan_s *p = nullptr;
+    int st = drop_in_plan(*mtxd, *ng, kgv, ekpg, isort, kmscr, *mxddim, *mxdgve, &p);
+    if (st) return st;
+    if (*neig == 0 || *mtxd == 0) return CPWGPU_OK;
+    if (!psi_sp || !hpsi_sp || !vscr_sp) return fail(CPWGPU_ERR_ARG, "null array argument");
+    if ((st = set_potential_spin(p, vscr_sp, *mxdscr, *nsp))) return st;
+    if ((st = set_projectors_spin_host(p, anlsp, xnlkbsp, *nanlsp, *mxddim))) return st;
+    const size_t colb = 32 * (size_t)(*mtxd);  // two components
+    if ((st = g_drop.psi.reserve(colb * *neig))) return st;
+    if ((st = g_drop.hpsi.reserve(colb * *neig))) return st;
+    HIPCHK(hipMemcpy2DAsync(g_drop.psi.p, colb, psi_sp, 32 * (size_t)*mxddim, colb, *neig, hipMemcpyHostToDevice,
+                            g_stream));
+    const double2 *dpsi = (const double2 *)g_drop.psi.p;
+    double2 *dhpsi = (double2 *)g_drop.hpsi.p;
+    const long ld = 2L * *mtxd;
+    st = g_prec == CPWGPU_PREC_FP32 ? apply_spin<float>(p, *neig, dpsi, ld, dhpsi, ld, g_stream)
+                                    : apply_spin<double>(p, *neig, dpsi, ld, dhpsi, ld, g_stream);
+    if (st) return st;
+    HIPCHK(hipMemcpy2DAsync(hpsi_sp, 32 * (size_t)*mxddim, g_drop.hpsi.p, colb, colb, *neig, hipMemcpyDeviceToHost,
+                            g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_hk_psi_c8(const int *mtxd, const int *neig, const float *psi, float *hpsi, const int *lnewanl,
+                     const int *ng, const int *kgv, const float *ekpg, const int *isort, const float *vscr,
+                     const int *kmscr, const float *anlga, const float *xnlkb, const int *nanl, const int *mxddim,
+                     const int *mxdbnd, const int *mxdanl, const int *mxdgve, const int *mxdscr) {
+    (void)lnewanl;
+    (void)mxdbnd;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!mtxd || !neig || !ng || !kmscr || !nanl || !mxddim || !mxdanl || !mxdgve || !mxdscr)
+        return fail(CPWGPU_ERR_ARG, "null scalar argument");
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", *ng, *mxdgve);
+    int ns[3], mfft, mwrk;
+    size_fft(kmscr, ns, &mfft, &mwrk);
+    if (mwrk > *mxdscr) return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", mwrk, *mxdscr);
+    if (*nanl > *mxdanl) return fail(CPWGPU_ERR_ARG, "nanl = %d > mxdanl = %d", *nanl, *mxdanl);
+    if (*neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", *neig);
+    if (*mtxd < 0 || *mtxd > *mxddim) return fail(CPWGPU_ERR_MTXD, "mtxd = %d, mxddim = %d", *mtxd, *mxddim);
+    if (*mtxd > 0 && !ekpg) return fail(CPWGPU_ERR_ARG, "null ekpg");
+    // the plan and the potential are kept in FP64 (exact images of the FP32 values)
+    std::vector<double> ek(ekpg, ekpg + std::max(*mtxd, 0));
+    cpwgpu_plan_s *p = nullptr;
+    int st = drop_in_plan(*mtxd, *ng, kgv, ek.data(), isort, kmscr, *mxddim, *mxdgve, &p);
+    if (st) return st;
+    if (*neig == 0 || *mtxd == 0) return CPWGPU_OK;
+    if (!psi || !hpsi || !vscr) return fail(CPWGPU_ERR_ARG, "null array argument");
+    std::vector<double> v(vscr, vscr + std::min<size_t>(p->ntot, (size_t)*mxdscr));
+    v.resize(std::max<size_t>(v.size(), p->ntot), 0.0);
+    if ((st = set_potential(p, v.data(), *mxdscr))) return st;
+    const int m = *mtxd, n = *neig, na = *nanl;
+    const size_t c8b = 8 * (size_t)m, c16b = 16 * (size_t)m;
+    const size_t stage = std::max((size_t)n, (size_t)na) * c8b;
+    if ((st = g_drop.c8.reserve(stage)) || (st = g_drop.psi.reserve(c16b * n)) || (st = g_drop.hpsi.reserve(c16b * n)))
+        return st;
+    auto grid = [](long tot) { return dim3((unsigned)((tot + 255) / 256)); };
+    // projectors: complex64 -> the plan's complex128 copy (the FP32 path rounds them back exactly);
+    // host staging vectors live until the final synchronisation
+    std::vector<double> x(xnlkb ? xnlkb : (const float *)nullptr, xnlkb ? xnlkb + na : (const float *)nullptr);
+    p->nanl = na;
+    p->anl32_valid = false;
+    if (na > 0) {
+        if (!anlga || !xnlkb) return fail(CPWGPU_ERR_ARG, "null projector array");
+        p->lda = m;
+        if ((st = p->anl64.reserve(c16b * na))) return st;
+        HIPCHK(hipMemcpy2DAsync(g_drop.c8.p, c8b, anlga, 8 * (size_t)*mxddim, c8b, na, hipMemcpyHostToDevice, g_stream));
+        hipLaunchKernelGGL(k_to_c128, grid((long)m * na), dim3(256), 0, g_stream, (const cplx<float> *)g_drop.c8.p,
+                           (long)m, (double2 *)p->anl64.p, (long)m, m, na);
+        if ((st = upload(p->xnl, x.data(), na))) return st;
+    }
+    HIPCHK(hipMemcpy2DAsync(g_drop.c8.p, c8b, psi, 8 * (size_t)*mxddim, c8b, n, hipMemcpyHostToDevice, g_stream));
+    hipLaunchKernelGGL(k_to_c128, grid((long)m * n), dim3(256), 0, g_stream, (const cplx<float> *)g_drop.c8.p,
+                       (long)m, (double2 *)g_drop.psi.p, (long)m, m, n);
+    HIPCHK(hipGetLastError());
+    if ((st = apply_t<float>(p, n, (const double *)g_drop.psi.p, m, (double *)g_drop.hpsi.p, m, g_stream))) return st;
+    hipLaunchKernelGGL(k_to_c64, grid((long)m * n), dim3(256), 0, g_stream, (const double2 *)g_drop.hpsi.p,
+                       (long)m, (cplx<float> *)g_drop.c8.p, (long)m, m, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy2DAsync(hpsi, 8 * (size_t)*mxddim, g_drop.c8.p, c8b, c8b, n, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
 }

@@ -703,6 +703,17 @@ __global__ void k_to_c64(const double2 *__restrict__ src, long lds, cplx<float> 
     dst[(long)c * ldd + r] = mk<float>((float)v.x, (float)v.y);
 }
 
+// complex64 (rows, cols; ld lds) -> complex128 (ld ldd): the inputs of the
+// single-precision drop-in
+__global__ void k_to_c128(const cplx<float> *__restrict__ src, long lds, double2 *__restrict__ dst, long ldd,
+                          int rows, int cols) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)rows * cols) return;
+    const int c = (int)(t / rows), r = (int)(t - (long)c * rows);
+    const cplx<float> v = src[(long)c * lds + r];
+    dst[(long)c * ldd + r] = make_double2((double)v.x, (double)v.y);
+}
+
 __global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) dst[t] = (float)src[t];

@@ -49,6 +49,8 @@ EXPORTS = (
     "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
     "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune",
     "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
+    "cpwgpu_hk_psi_spin_c16", "cpwgpu_plan_set_potential_spin", "cpwgpu_plan_set_projectors_spin",
+    "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
@@ -106,6 +108,9 @@ def lib():
         L.cpwgpu_plan_tune.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_int]
         L.cpwgpu_plan_density.argtypes = [vp, C.c_int, dp, C.c_long, dp, dp, vp]
+        L.cpwgpu_plan_set_potential_spin.argtypes = [vp, dp, C.c_int, C.c_int]
+        L.cpwgpu_plan_set_projectors_spin.argtypes = [vp, dp, dp, C.c_int, C.c_int]
+        L.cpwgpu_plan_apply_spin.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_plan_ditsp.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                         dp, C.c_long, dp, dp, C.c_long, dp, C.c_long, dp, ip, ip, vp]
         _lib = L
@@ -156,6 +161,49 @@ def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr
         _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr))
     _check(st)
     return hpsi
+
+
+def hk_psi_c8(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr, anlga, xnlkb,
+              nanl, mxddim, mxdbnd, mxdanl, mxdgve, mxdscr):
+    """Single-precision H|psi> (cpwgpu_hk_psi_c8): the arguments of hk_psi_c16
+    with complex64 psi/hpsi/anlga and float32 ekpg/vscr/xnlkb (numpy, Fortran
+    layout).  hpsi is written in place."""
+    for name, arr, dt in (("psi", psi, np.complex64), ("hpsi", hpsi, np.complex64),
+                          ("kgv", kgv, np.int32), ("ekpg", ekpg, np.float32),
+                          ("isort", isort, np.int32), ("vscr", vscr, np.float32),
+                          ("kmscr", kmscr, np.int32), ("anlga", anlga, np.complex64),
+                          ("xnlkb", xnlkb, np.float32)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    lnew = C.c_int(1 if lnewanl else 0)
+    _check(lib().cpwgpu_hk_psi_c8(
+        _iref(mtxd), _iref(neig), _p(psi), _p(hpsi), C.byref(lnew), _iref(ng), _p(kgv), _p(ekpg),
+        _p(isort), _p(vscr), _p(kmscr), _p(anlga), _p(xnlkb), _iref(nanl), _iref(mxddim),
+        _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr)))
+    return hpsi
+
+
+def hk_psi_spin_c16(mtxd, neig, psi_sp, hpsi_sp, lnewanl, ng, kgv, ekpg, isort, vscr_sp, kmscr,
+                    nsp, anlsp, xnlkbsp, nanlsp, mxddim, mxdbnd, mxdasp, mxdgve, mxdscr, mxdnsp):
+    """Two-component H|psi> with the arguments of
+    Src/hk_psi/reference/hk_psi_spin_c16.f90:28-32 (numpy, Fortran layout):
+    psi_sp/hpsi_sp complex128 (2*mxddim, >=neig) with up/down rows
+    interleaved, vscr_sp float64 (mxdscr, mxdnsp), anlsp complex128
+    (2*mxddim, mxdasp).  hpsi_sp is written in place."""
+    for name, arr, dt in (("psi_sp", psi_sp, np.complex128), ("hpsi_sp", hpsi_sp, np.complex128),
+                          ("kgv", kgv, np.int32), ("ekpg", ekpg, np.float64),
+                          ("isort", isort, np.int32), ("vscr_sp", vscr_sp, np.float64),
+                          ("kmscr", kmscr, np.int32), ("anlsp", anlsp, np.complex128),
+                          ("xnlkbsp", xnlkbsp, np.float64)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    lnew = C.c_int(1 if lnewanl else 0)
+    _check(lib().cpwgpu_hk_psi_spin_c16(
+        _iref(mtxd), _iref(neig), _p(psi_sp), _p(hpsi_sp), C.byref(lnew), _iref(ng), _p(kgv),
+        _p(ekpg), _p(isort), _p(vscr_sp), _p(kmscr), _iref(nsp), _p(anlsp), _p(xnlkbsp),
+        _iref(nanlsp), _iref(mxddim), _iref(mxdbnd), _iref(mxdasp), _iref(mxdgve), _iref(mxdscr),
+        _iref(mxdnsp)))
+    return hpsi_sp
 
 
 def ditsp_c16(ipr, icmax, iguess, epspsi, lnewanl, neig, mtxd, mtxds, psi, hpsi, ei, ekpg,
@@ -336,6 +384,31 @@ class Plan:
         _check(lib().cpwgpu_plan_apply(self._h, int(neig), C.c_void_p(psi_t.data_ptr()),
                                        _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
                                        int(prec), C.c_void_p(stream)))
+        return hpsi_t
+
+    def set_potential_spin(self, vscr_sp, nsp):
+        """Potential components vscr_sp (mxdscr, >=nsp), nsp = 1, 2 or 4."""
+        v = np.asfortranarray(vscr_sp, np.float64)
+        _check(lib().cpwgpu_plan_set_potential_spin(self._h, _p(v), int(v.shape[0]), int(nsp)))
+
+    def set_projectors_spin(self, anlsp, xnlkbsp, nanlsp=None):
+        """Projectors on the interleaved rows: anlsp (2*mxddim, >=nanlsp)."""
+        a = np.asfortranarray(anlsp, np.complex128)
+        x = np.ascontiguousarray(xnlkbsp, np.float64)
+        nanlsp = a.shape[1] if nanlsp is None else nanlsp
+        _check(lib().cpwgpu_plan_set_projectors_spin(self._h, _p(a), _p(x), int(nanlsp),
+                                                     int(a.shape[0] // 2)))
+
+    def apply_spin(self, psi_t, hpsi_t, prec=PREC_FP64, stream=None):
+        """H psi for two-component bands: complex128 device tensors (neig, ld),
+        ld >= 2*mtxd, up/down rows interleaved."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        _check(lib().cpwgpu_plan_apply_spin(self._h, int(psi_t.shape[0]), C.c_void_p(psi_t.data_ptr()),
+                                            _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
+                                            int(prec), C.c_void_p(stream)))
         return hpsi_t
 
     def density(self, psi_t, occ=None, stream=None):

@@ -385,3 +385,54 @@ def make_config(cfg, nbands: int | None = None, kindex: int = 0, with_psi: bool 
                 species=np.concatenate([np.full(t[1].shape[0], k, np.int32)
                                         for k, t in enumerate(spec.types)]),
                 proj_atom=projector_atoms(spec), floc_q=floc_q, floc_f=floc_f)
+
+
+# --------------------------------------------------------------------------
+# two-component (spinor) inputs of hk_psi_spin_c16
+# --------------------------------------------------------------------------
+
+def spin_inputs(d, nbands, seed=11):
+    """Inputs of Src/hk_psi/reference/hk_psi_spin_c16.f90 built on a scalar
+    config ``d``:
+
+    * ``vscr_sp`` (mxdscr, 4): component 1 is d's V(r); components 2-4
+      (sigma_z, sigma_x, sigma_y parts) are further band-limited random
+      potentials at 0.3 of its amplitude, so every coupling is exercised;
+    * ``anlsp`` (2*mxddim, 2*nanl), ``xnlkbsp``: each scalar projector beta_a
+      gives the pair beta_a (x) U|up>, beta_a (x) U|down> (interleaved rows,
+      U a fixed SU(2) rotation) with strengths xnlkb_a and 0.5*xnlkb_a, a
+      spin-dependent non-local term like a spin-orbit one;
+    * ``psi_sp`` (2*mxddim, nbands): two random wave-function sets
+      interleaved, normalised per band."""
+    spec = d["spec"]
+    ns = [int(x) for x in d["kmscr"][3:6]]
+    idd = int(d["kmscr"][6])
+    rng = np.random.Generator(np.random.PCG64(seed))
+    mxdscr = d["mxdscr"]
+    vsp = np.zeros((mxdscr, 4), order="F")
+    vsp[:, 0] = d["vscr"]
+    for c in range(1, 4):
+        gq, vq = _random_potential(rng, spec, d["kgv"], d["bvec"], ns)
+        v = potential_on_mesh(gq, 0.3 * vq, ns, idd)
+        vsp[: v.shape[0], c] = v
+    mtxd, mxddim, nanl = d["mtxd"], d["mxddim"], d["nanl"]
+    th, ph = 0.7, 1.3
+    U = np.array([[np.cos(th / 2), -np.exp(-1j * ph) * np.sin(th / 2)],
+                  [np.exp(1j * ph) * np.sin(th / 2), np.cos(th / 2)]])
+    anlsp = np.zeros((2 * mxddim, 2 * nanl), np.complex128, order="F")
+    xsp = np.zeros(2 * nanl)
+    for a in range(nanl):
+        b = d["anlga"][:mtxd, a]
+        for s in range(2):
+            col = 2 * a + s
+            anlsp[0:2 * mtxd:2, col] = b * U[0, s]
+            anlsp[1:2 * mtxd:2, col] = b * U[1, s]
+            xsp[col] = d["xnlkb"][a] * (1.0 if s == 0 else 0.5)
+    w = np.exp(-d["ekpg"][:mtxd] / 4)
+    psi = np.zeros((2 * mxddim, nbands), np.complex128, order="F")
+    for c in range(2):
+        x = rng.normal(size=(mtxd, nbands)) + 1j * rng.normal(size=(mtxd, nbands))
+        psi[c:2 * mtxd:2] = x * w[:, None]
+    psi /= np.linalg.norm(psi, axis=0)
+    return dict(vscr_sp=vsp, anlsp=anlsp, xnlkbsp=xsp, nanlsp=2 * nanl, psi_sp=psi,
+                mxdasp=2 * nanl, mxdnsp=4)

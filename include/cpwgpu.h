@@ -80,6 +80,35 @@ int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, doubl
                       const double *xnlkb, const int *nanl, const int *mxddim, const int *mxdbnd,
                       const int *mxdanl, const int *mxdgve, const int *mxdscr);
 
+/* Replaces Src/hk_psi/reference/hk_psi_spin_c16.f90:28-307 (subroutine
+ * hk_psi_spin_c16), the two-component version: same dummies, by reference,
+ *   psi_sp(2*mxddim, neig), hpsi_sp(2*mxddim, neig) complex(8): row 2i-1 / 2i
+ *                       (1-based) = spin up / down coefficient of plane wave i
+ *   vscr_sp(mxdscr, mxdnsp) real(8): V(r) components (1, sigma_z, sigma_x,
+ *                       sigma_y); nsp = 1, 2 or 4 of them are used
+ *   anlsp(2*mxddim, mxdasp) complex(8), xnlkbsp(mxdasp), nanlsp: projectors on
+ *                       the interleaved rows
+ * The reference's stops (ng > mxdgve, nsp > mxdnsp, nsp not 1/2/4, mxdwrk >
+ * mxdscr, wrap-around) return CPWGPU_ERR_NG / _ARG / _MODE / _MXDSCR / _WRAP.
+ * Precision as set by cpwgpu_set_precision. */
+int cpwgpu_hk_psi_spin_c16(const int *mtxd, const int *neig, const double *psi_sp, double *hpsi_sp,
+                           const int *lnewanl, const int *ng, const int *kgv, const double *ekpg, const int *isort,
+                           const double *vscr_sp, const int *kmscr, const int *nsp, const double *anlsp,
+                           const double *xnlkbsp, const int *nanlsp, const int *mxddim, const int *mxdbnd,
+                           const int *mxdasp, const int *mxdgve, const int *mxdscr, const int *mxdnsp);
+
+/* Single-precision drop-in, the complex*8 interface of the reference's
+ * Src/hk_psi/cuda/hk_psi_c8.f90 (hk_psi_c8) brought to the current argument
+ * list: the dummies of hk_psi_c16 with complex(4) psi, hpsi, anlga and real(4)
+ * ekpg, vscr, xnlkb.  Computed on the FP32-mixed path (inputs widened
+ * exactly, H psi rounded to complex(4) on the way out); same checks and
+ * status codes as cpwgpu_hk_psi_c16.  The reference's stale version takes
+ * qmod (|k+G|) and kmscr(3) instead of ekpg and kmscr(7). */
+int cpwgpu_hk_psi_c8(const int *mtxd, const int *neig, const float *psi, float *hpsi, const int *lnewanl,
+                     const int *ng, const int *kgv, const float *ekpg, const int *isort, const float *vscr,
+                     const int *kmscr, const float *anlga, const float *xnlkb, const int *nanl, const int *mxddim,
+                     const int *mxdbnd, const int *mxdanl, const int *mxdgve, const int *mxdscr);
+
 /* ---- device-resident plan ------------------------------------------------ */
 
 /* Builds the index maps of one k-point from host arrays (the hk_psi_c16
@@ -101,6 +130,17 @@ int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_de
  * stream).  Returns when the work is enqueued. */
 int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
                       double *hpsi_dev, long ldh, int prec, void *stream);
+/* Two-component bands on the same plan (the device side of
+ * cpwgpu_hk_psi_spin_c16): the potential components vscr_sp(mxdscr, nsp) and
+ * the projectors anlsp(2*mxddim, nanlsp) from host memory, then H psi_sp on
+ * device arrays psi_dev(ldp, neig), hpsi_dev(ldh, neig) with ldp, ldh >=
+ * 2*mtxd (interleaved up/down rows).  nsp = 4 (non-collinear) needs one of
+ * the specialised meshes. */
+int cpwgpu_plan_set_potential_spin(cpwgpu_plan_t plan, const double *vscr_sp, int mxdscr, int nsp);
+int cpwgpu_plan_set_projectors_spin(cpwgpu_plan_t plan, const double *anlsp, const double *xnlkbsp, int nanlsp,
+                                    int mxddim);
+int cpwgpu_plan_apply_spin(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                           long ldh, int prec, void *stream);
 /* Sizes of the plan: out[0..9] = mtxd, n1, n2, n3, id, kd1, kd2, kd3, ncol, nanl. */
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 

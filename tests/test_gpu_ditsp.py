@@ -173,3 +173,30 @@ def test_si64_fp64_and_fp32_mixed_agree(torch_dev, hk):
         out[prec] = ei
     assert np.abs(out[hk.PREC_FP64] - out[hk.PREC_FP32]).max() < 1e-9
     plan.close()
+
+
+def test_fortran_shim_ditsp_c16(hk, cell2):
+    """The Fortran drop-in ditsp_c16_ (libcpwgpu_fortran.so), by reference."""
+    import ctypes as C
+
+    from tests.test_host import _ditsp_args
+
+    d, h, exact = cell2
+    neig = 8
+    a = _ditsp_args(d, neig, epspsi=EPS, icmax=100)
+    ms = a["mtxds"]
+    a["hamsm"][:ms, :ms] = h[:ms, :ms]
+    a["hdiag"][: d["mtxd"]] = np.real(np.diag(h))
+    lib = C.CDLL(hk.FLIB_PATH)
+    i = lambda v: C.byref(C.c_int(int(v)))
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    ifail = C.c_int(-7)
+    lnew = C.c_int(1)
+    nref = C.c_int(neig)
+    lib.ditsp_c16_(i(0), C.byref(ifail), i(a["icmax"]), i(0), C.byref(C.c_double(EPS)), C.byref(lnew),
+                   C.byref(nref), i(d["mtxd"]), i(ms), p(a["psi"]), p(a["hpsi"]), p(a["ei"]), p(a["ekpg"]),
+                   p(a["isort"]), p(a["vscr"]), p(a["kmscr"]), i(a["ng"]), p(a["kgv"]), p(a["anlga"]),
+                   p(a["xnlkb"]), i(a["nanl"]), p(a["hamsm"]), p(a["hdiag"]), i(a["mxddim"]), i(a["mxdsml"]),
+                   i(a["mxdbnd"]), i(a["mxdgve"]), i(a["mxdscr"]), i(a["mxdanl"]))
+    assert ifail.value == 0 and lnew.value == 1 and nref.value == neig
+    assert np.abs(a["ei"][:neig] - exact[:neig]).max() < 1e-9

@@ -80,6 +80,7 @@ def test_fortran_shim_exports_hk_psi_c16_and_ditsp_c16():
                          text=True, check=True).stdout
     assert re.search(r"\bT hk_psi_c16_\b", out)
     assert re.search(r"\bT ditsp_c16_\b", out)
+    assert re.search(r"\bT hk_psi_spin_c16_\b", out)
 
 
 def _ditsp_args(d, neig, **over):
@@ -145,3 +146,30 @@ def test_argument_errors_before_any_device_work():
     with pytest.raises(hk_psi.CpwGpuError) as e:
         hk_psi.set_precision(7)
     assert e.value.code == hk_psi.ERR_MODE
+
+
+def test_hk_psi_spin_argument_checks_of_the_reference():
+    """cpwgpu_hk_psi_spin_c16 reports the stops of
+    Src/hk_psi/reference/hk_psi_spin_c16.f90:106-141 before any device work."""
+    from cpw2000_amd import hk_psi
+
+    d = synth.make_config(0, nbands=2)
+    s = synth.spin_inputs(d, 2)
+
+    def call(nsp=4, mxdnsp=4, mxdgve=None, mxdscr=None):
+        hpsi = np.zeros_like(s["psi_sp"], order="F")
+        hk_psi.hk_psi_spin_c16(d["mtxd"], 2, s["psi_sp"], hpsi, True, d["ng"],
+                               np.asfortranarray(d["kgv"], np.int32), d["ekpg"],
+                               np.asfortranarray(d["isort"], np.int32), s["vscr_sp"],
+                               np.asfortranarray(d["kmscr"], np.int32), nsp, s["anlsp"],
+                               s["xnlkbsp"], s["nanlsp"], d["mxddim"], 2, s["mxdasp"],
+                               d["mxdgve"] if mxdgve is None else mxdgve,
+                               d["mxdscr"] if mxdscr is None else mxdscr, mxdnsp)
+
+    for kw, code in ((dict(mxdgve=d["ng"] - 1), hk_psi.ERR_NG),
+                     (dict(nsp=4, mxdnsp=2), hk_psi.ERR_ARG),
+                     (dict(nsp=3), hk_psi.ERR_MODE),
+                     (dict(mxdscr=100), hk_psi.ERR_MXDSCR)):
+        with pytest.raises(hk_psi.CpwGpuError) as e:
+            call(**kw)
+        assert e.value.code == code, kw

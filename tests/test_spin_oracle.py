@@ -1,0 +1,97 @@
+"""The two-component oracle (oracle/spin.py) pinned without the GPU:
+
+* nsp = 1, 2 with spin-diagonal projectors: each component equals the scalar
+  C restatement (pinned to the reference's golden vectors) with V1, V1 + V2,
+  V1 - V2;
+* nsp = 4: equals the dense spinor H built in G space, to 1e-11 relative;
+* constant fields, no projectors: the action at each plane wave is
+  ekpg + V1 + V2 sigma_z + V3 sigma_x + V4 sigma_y with the Pauli matrices;
+* Hermiticity of the nsp = 4 operator with spin-mixing projectors."""
+
+import numpy as np
+import pytest
+
+from cpw2000_amd import synth
+from oracle import oracle as orc
+from oracle import spin
+
+
+@pytest.fixture(scope="module")
+def case():
+    d = synth.make_config(0, nbands=2)
+    return d, synth.spin_inputs(d, 5)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _diag_projectors(d):
+    mtxd, mxddim, nanl = d["mtxd"], d["mxddim"], d["nanl"]
+    a = np.zeros((2 * mxddim, 2 * nanl), np.complex128, order="F")
+    a[0:2 * mtxd:2, 0::2] = d["anlga"][:mtxd, :nanl]
+    a[1:2 * mtxd:2, 1::2] = d["anlga"][:mtxd, :nanl]
+    return a, np.repeat(d["xnlkb"][:nanl], 2)
+
+
+@pytest.mark.parametrize("nsp", [1, 2])
+def test_collinear_components_equal_the_scalar_oracle(case, nsp):
+    d, s = case
+    mtxd, mxddim = d["mtxd"], d["mxddim"]
+    a, x = _diag_projectors(d)
+    psi = s["psi_sp"]
+    h = spin.hk_psi_spin(d, psi, s["vscr_sp"], nsp, a, x, a.shape[1])
+    v1, v2 = s["vscr_sp"][:, 0], s["vscr_sp"][:, 1]
+    vc = [v1, v1] if nsp == 1 else [v1 + v2, v1 - v2]
+    for c in range(2):
+        comp = np.zeros((mxddim, psi.shape[1]), np.complex128, order="F")
+        comp[:mtxd] = psi[c:2 * mtxd:2]
+        st, hs = orc.orc_hk_psi(dict(d, vscr=np.ascontiguousarray(vc[c])), psi=comp)
+        assert st == 0
+        assert _rel(h[c::2], hs[:mtxd]) < 1e-12
+
+
+def test_noncollinear_equals_dense_spinor_h(case):
+    d, s = case
+    mtxd = d["mtxd"]
+    args = (s["vscr_sp"], 4, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
+    h = spin.hk_psi_spin(d, s["psi_sp"], *args)
+    hd = spin.dense_h_spin(d, *args)
+    assert np.abs(hd - hd.conj().T).max() < 1e-12
+    assert _rel(h, hd @ s["psi_sp"][: 2 * mtxd]) < 1e-11
+
+
+def test_constant_fields_give_the_pauli_form(case):
+    d, s = case
+    mtxd = d["mtxd"]
+    a, b, c, e = -0.4, 0.25, -0.15, 0.35
+    v = np.zeros_like(s["vscr_sp"])
+    v[:, 0], v[:, 1], v[:, 2], v[:, 3] = a, b, c, e
+    psi = s["psi_sp"]
+    h = spin.hk_psi_spin(d, psi, v, 4)
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sy = np.array([[0, -1j], [1j, 0]])
+    sz = np.array([[1, 0], [0, -1]], complex)
+    m = a * np.eye(2) + b * sz + c * sx + e * sy
+    ek = d["ekpg"][:mtxd]
+    for n in range(psi.shape[1]):
+        u, dn = psi[0:2 * mtxd:2, n], psi[1:2 * mtxd:2, n]
+        want_u = ek * u + m[0, 0] * u + m[0, 1] * dn
+        want_d = ek * dn + m[1, 0] * u + m[1, 1] * dn
+        assert np.abs(h[0::2, n] - want_u).max() < 1e-12
+        assert np.abs(h[1::2, n] - want_d).max() < 1e-12
+    # spectrum of one plane wave: ekpg + a +/- |(b, c, e)|
+    w = np.linalg.eigvalsh(spin.dense_h_spin(d, v, 4))
+    r = np.sqrt(b * b + c * c + e * e)
+    want = np.sort(np.concatenate([ek + a - r, ek + a + r]))
+    assert np.abs(w - want).max() < 1e-12
+
+
+def test_noncollinear_operator_is_hermitian(case):
+    d, s = case
+    mtxd = d["mtxd"]
+    args = (s["vscr_sp"], 4, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
+    psi = s["psi_sp"][: 2 * mtxd]
+    h = spin.hk_psi_spin(d, s["psi_sp"], *args)
+    g = psi.conj().T @ h
+    assert np.abs(g - g.conj().T).max() < 1e-12 * np.abs(g).max()
