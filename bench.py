@@ -79,6 +79,26 @@ def roofline_entry(name, work, avg_ms, nb, prec):
             "avg_ms": round(avg_ms, 5), "algorithmic_flops": w["flops"] * nb}
 
 
+def call_roofline(info, nanl, nb, prec, ms):
+    """Whole-call roofline of one H|psi> over nb bands: the local stage at the
+    HBM bound (per band s_c*2*mtxd for psi in and H psi out, 4*s_c*ncol*n3 for
+    the z-column round trips of the two 3-D transforms, s_r*N for V(r)) plus the
+    two projector GEMMs at the matrix-core bound (16*mtxd*nanl flop per band);
+    frac = t_roof / t_measured."""
+    s_c = 16 if prec == "fp64" else 8
+    s_r = 8 if prec == "fp64" else 4
+    mtxd, ncol = info["mtxd"], info["ncol"]
+    n = info["n1"] * info["n2"] * info["n3"]
+    b_loc = s_c * 2 * mtxd + 4 * s_c * ncol * info["n3"] + s_r * n
+    f_nl = 16 * mtxd * nanl
+    t_loc = b_loc * nb / (PEAK_HBM_GBS * 1e9) * 1e6
+    t_nl = f_nl * nb / (PEAK_TFLOPS[prec] * 1e12) * 1e6
+    return {"t_roof_us": round(t_loc + t_nl, 1), "t_local_us": round(t_loc, 1),
+            "t_nonlocal_us": round(t_nl, 1), "t_measured_us": round(ms * 1e3, 1),
+            "frac": round((t_loc + t_nl) / (ms * 1e3), 4),
+            "roofline_bands_per_s": round(nb / ((t_loc + t_nl) * 1e-6), 1)}
+
+
 def measured_traffic(prec, plane_variant, n1):
     """HBM bytes per launch of the plane-stage kernel measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on this
@@ -411,6 +431,7 @@ def main():
                                    nb, "fp32"),
         "roofline_fp64": roofline_entry(res["fp64"]["dominant"], work["fp64"],
                                         res["fp64"]["dom_ms"], nb, "fp64"),
+        "call_roofline": {p: call_roofline(info, nanl, nb, p, res[p]["ms"]) for p in ("fp32", "fp64")},
         "tuned": tuned,
         "kernel_ms": kernels,
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
