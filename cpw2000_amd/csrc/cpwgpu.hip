@@ -1100,7 +1100,7 @@ int ensure_blas(hipStream_t st) {
 }
 
 struct DitspWs {
-    DevBuf xa, hxa, xb, hxb, pb, cpk, hd, m, g, g0, pm, w, e, info, res, act, d, mones;
+    DevBuf xa, hxa, xb, hxb, pb, cpk, hd, m, g, g0, pm, w, e, info, res, act, d, mones, eall;
 } g_dt;
 
 constexpr double DT_PIVOT = 1e-10;      // squared norm a unit column must keep outside the span
@@ -1303,7 +1303,8 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
         (rc = g_dt.pb.reserve(col * nb)) || (rc = g_dt.cpk.reserve(16 * (size_t)2 * nb * nb)) ||
         (rc = g_dt.m.reserve(16 * (size_t)mdim * mdim)) || (rc = g_dt.w.reserve(8 * (size_t)mdim)) ||
         (rc = g_dt.e.reserve(8 * (size_t)mdim)) || (rc = g_dt.info.reserve(64)) ||
-        (rc = g_dt.res.reserve(8 * (size_t)nb)) || (rc = g_dt.act.reserve(4 * (size_t)nb)))
+        (rc = g_dt.res.reserve(8 * (size_t)nb)) || (rc = g_dt.act.reserve(8 * (size_t)nb)) ||
+        (rc = g_dt.eall.reserve(8 * (size_t)nb)))
         return rc;
     double2 *X = (double2 *)g_dt.xa.p, *HX = (double2 *)g_dt.hxa.p;
     double2 *XB = (double2 *)g_dt.xb.p, *HXB = (double2 *)g_dt.hxb.p, *PB = (double2 *)g_dt.pb.p;
@@ -1384,16 +1385,28 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
     }
     int pc = prec;
     bool refined = prec == CPWGPU_PREC_FP64;
+    // Locked vectors: X(:, 0:nl), the lowest converged ones (their residual^2
+    // below 0.01 epspsi^2); they stay fixed, new directions are still made
+    // orthogonal to them, and each Rayleigh-Ritz step only diagonalises
+    // [X(:, nl:nb), new block].  E holds the Ritz value of every column of X.
+    int nl = 0;
     bool havep = false;
-    auto ritz_nb = [&](int nt) -> int {
-        int r = ritz_step(mtxd, X, HX, nt, nb, XB, HXB, nt > nb ? PB : nullptr, st);
-        havep = nt > nb;
+    int pbase = 0;  // column j of PB is the direction of vector pbase + j
+    double *E = (double *)g_dt.eall.p;
+    int *dact = (int *)g_dt.act.p, *dactp = dact + nb;
+    auto ritz_free = [&](int kept) -> int {
+        const int nu = nb - nl;
+        double2 *Xu = X + (size_t)nl * mtxd, *HXu = HX + (size_t)nl * mtxd;
+        int r = ritz_step(mtxd, Xu, HXu, nu + kept, nu, XB, HXB, kept > 0 ? PB : nullptr, st);
         if (r) return r;
-        HIPCHK(hipMemcpyAsync(X, XB, col * nb, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(HX, HXB, col * nb, hipMemcpyDeviceToDevice, st));
+        havep = kept > 0;
+        pbase = nl;
+        HIPCHK(hipMemcpyAsync(Xu, XB, col * nu, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(HXu, HXB, col * nu, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(E + nl, W, 8 * (size_t)nu, hipMemcpyDeviceToDevice, st));
         return CPWGPU_OK;
     };
-    if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_nb(nb))) return rc;
+    if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_free(0))) return rc;
     const double eps2 = epspsi * epspsi;
     const double sw2 = std::max(eps2, DT_SWITCH2);
     // CPWGPU_DITSP_TRACE=1: one line per iteration on stderr (the ipr > 3
@@ -1402,35 +1415,40 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
         const char *e = getenv("CPWGPU_DITSP_TRACE");
         return e && *e && *e != '0';
     }();
-    std::vector<double> res(nb), wv(nb);
-    std::vector<int> act;
+    std::vector<double> res(nb, HUGE_VAL), wv(nb);
+    std::vector<int> act, actp;
     act.reserve(nb);
+    actp.reserve(nb);
     double xmax = 0.0, xprev = HUGE_VAL;
     bool conv = false;
     for (int ic = 0;; ic++) {
-        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, (const double2 *)X, (long)mtxd,
-                           (const double2 *)HX, (long)mtxd, (const double *)W, (double *)g_dt.res.p, mtxd);
+        const int nu = nb - nl;
+        // residuals of the unlocked vectors; the locked ones keep theirs
+        hipLaunchKernelGGL(k_residual, dim3(nu), dim3(256), 0, st, (const double2 *)(X + (size_t)nl * mtxd),
+                           (long)mtxd, (const double2 *)(HX + (size_t)nl * mtxd), (long)mtxd,
+                           (const double *)(E + nl), (double *)g_dt.res.p + nl, mtxd);
         HIPCHK(hipGetLastError());
         int info = 0;
-        HIPCHK(hipMemcpyAsync(res.data(), g_dt.res.p, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(wv.data(), W, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(res.data() + nl, (double *)g_dt.res.p + nl, 8 * (size_t)nu, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(wv.data(), E, 8 * (size_t)nb, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(&info, dinfo + 1, sizeof info, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (info) return fail(CPWGPU_ERR_HIP, "zheevd of the subspace failed (info %d)", info);
         iters = ic;
         xmax = *std::max_element(res.begin(), res.begin() + neig);
         if (trace)
-            fprintf(stderr, "ditsp: ic %4d  max res^2 %.3e  e[0] %.10f  e[neig-1] %.10f  %s\n", ic, xmax, wv[0],
-                    wv[neig - 1], refined ? "fp64" : "fp32");
+            fprintf(stderr, "ditsp: ic %4d  max res^2 %.3e  locked %d  %s\n", ic, xmax, nl, refined ? "fp64" : "fp32");
         if (!refined) {
             // the FP32 products stop helping near their rounding level: switch
             // there, or once the residual stalls below 1e-6
             if (xmax < sw2 || (xmax < 1e-6 && xmax > 0.9 * xprev)) {
-                // FP32-mixed: FP64 H on the current vectors, then FP64 iterations
+                // FP32-mixed: unlock everything, FP64 H on the current vectors, then FP64 iterations
                 refined = true;
                 refine_at = ic;
                 pc = CPWGPU_PREC_FP64;
-                if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_nb(nb))) return rc;
+                nl = 0;
+                std::fill(res.begin(), res.end(), HUGE_VAL);
+                if ((rc = happly(pc, X, HX, nb)) || (rc = ritz_free(0))) return rc;
                 xprev = HUGE_VAL;
                 continue;
             }
@@ -1439,25 +1457,30 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
             break;
         }
         if (ic >= icmax) break;
-        // active vectors, largest residual first
         const double thr = 0.01 * (refined ? eps2 : sw2);
+        // lock the lowest unlocked vectors that have converged
+        while (nl < nb - 1 && res[nl] <= thr) nl++;
+        // active vectors (unlocked, not converged), largest residual first
         act.clear();
-        for (int n = 0; n < nb; n++)
+        actp.clear();
+        for (int n = nl; n < nb; n++)
             if (res[n] > thr) act.push_back(n);
         std::stable_sort(act.begin(), act.end(), [&](int a, int b) { return res[a] > res[b]; });
         const int m = (int)act.size();
         if (m == 0) break;
-        HIPCHK(hipMemcpyAsync(g_dt.act.p, act.data(), 4 * (size_t)m, hipMemcpyHostToDevice, st));
+        for (int n : act) actp.push_back(n - pbase);
+        HIPCHK(hipMemcpyAsync(dact, act.data(), 4 * (size_t)m, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dactp, actp.data(), 4 * (size_t)m, hipMemcpyHostToDevice, st));
         double2 *T = X + (size_t)nb * mtxd, *HT = HX + (size_t)nb * mtxd;
         hipLaunchKernelGGL(k_precondition, dt_grid(mtxd, m), dim3(DT_THREADS), 0, st, (const double2 *)X,
-                           (long)mtxd, (const double2 *)HX, (long)mtxd, (const double *)W, (const double *)hd,
-                           (const int *)g_dt.act.p, T, (long)mtxd, mtxd, m);
+                           (long)mtxd, (const double2 *)HX, (long)mtxd, (const double *)E, (const double *)hd,
+                           (const int *)dact, T, (long)mtxd, mtxd, m);
         // previous search directions of the active vectors (the third block
         // of a locally optimal block preconditioned iteration)
         int mt = m;
         if (havep) {
             hipLaunchKernelGGL(k_gather_cols, dt_grid(mtxd, m), dim3(DT_THREADS), 0, st, (const double2 *)PB,
-                               (long)mtxd, (const int *)g_dt.act.p, T + (size_t)m * mtxd, (long)mtxd, mtxd, m);
+                               (long)mtxd, (const int *)dactp, T + (size_t)m * mtxd, (long)mtxd, mtxd, m);
             mt = 2 * m;
         }
         hipLaunchKernelGGL(k_normalize_cols, dim3(mt), dim3(DT_THREADS), 0, st, T, (long)mtxd, mtxd);
@@ -1465,13 +1488,28 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
         if ((rc = orthonormalize(mtxd, X, mtxd, nb, T, mtxd, mt, &kept, st))) return rc;
         if (trace) fprintf(stderr, "ditsp:        active %d  new directions %d of %d\n", m, kept, mt);
         if (kept == 0) break;  // no direction left outside the current vectors
-        if ((rc = happly(pc, T, HT, kept)) || (rc = ritz_nb(nb + kept))) return rc;
+        if ((rc = happly(pc, T, HT, kept)) || (rc = ritz_free(kept))) return rc;
         xprev = xmax;
     }
-    HIPCHK(hipMemcpy2DAsync(psi, 16 * (size_t)ldp, X, col, col, neig, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)ldh, HX, col, col, neig, hipMemcpyDeviceToDevice, st));
+    // the neig lowest pairs in ascending order (a vector found late can lie below a locked one)
+    std::vector<int> order(nb);
+    for (int n = 0; n < nb; n++) order[n] = n;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wv[a] < wv[b]; });
+    bool ident = true;
+    for (int n = 0; n < neig; n++) ident = ident && order[n] == n;
+    if (ident) {
+        HIPCHK(hipMemcpy2DAsync(psi, 16 * (size_t)ldp, X, col, col, neig, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)ldh, HX, col, col, neig, hipMemcpyDeviceToDevice, st));
+    } else {
+        HIPCHK(hipMemcpyAsync(dact, order.data(), 4 * (size_t)neig, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_gather_cols, dt_grid(mtxd, neig), dim3(DT_THREADS), 0, st, (const double2 *)X,
+                           (long)mtxd, (const int *)dact, psi, ldp, mtxd, neig);
+        hipLaunchKernelGGL(k_gather_cols, dt_grid(mtxd, neig), dim3(DT_THREADS), 0, st, (const double2 *)HX,
+                           (long)mtxd, (const int *)dact, hpsi, ldh, mtxd, neig);
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(st));
-    std::copy(wv.begin(), wv.begin() + neig, ei);
+    for (int n = 0; n < neig; n++) ei[n] = wv[order[n]];
     finish(conv, xmax);
     return CPWGPU_OK;
 }
@@ -1829,7 +1867,7 @@ int cpwgpu_finalize(void) {
     for (DevBuf *b : {&g_dt.xa, &g_dt.hxa, &g_dt.xb, &g_dt.hxb, &g_dt.pb, &g_dt.cpk, &g_dt.hd, &g_dt.m,
                       &g_dt.g, &g_dt.g0,
                       &g_dt.pm, &g_dt.w, &g_dt.e, &g_dt.info, &g_dt.res, &g_dt.act, &g_dt.d, &g_dt.mones,
-                      &g_fw.rhor})
+                      &g_dt.eall, &g_fw.rhor})
         b->release();
     if (g_aux) {
         HIPCHK(hipStreamSynchronize(g_aux));
