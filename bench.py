@@ -279,6 +279,44 @@ def bench_ditsp(torch, hk, plan, d, nb, np):
     return out
 
 
+def bench_spin_density(torch, hk, synth, np, d, steps):
+    """Two-component H|psi> (non-collinear potential, nsp = 4, spin-mixing
+    projectors) on 64 spinor bands of the 64-atom cell, and the electron
+    density of its 128 scalar bands; device arrays, per-call time."""
+    mtxd = d["mtxd"]
+    s = synth.spin_inputs(d, 64)
+    plan = hk.Plan.from_inputs(d, projectors=False)
+    plan.set_potential_spin(s["vscr_sp"], 4)
+    plan.set_projectors_spin(s["anlsp"], s["xnlkbsp"], s["nanlsp"])
+    pt = torch.from_numpy(np.ascontiguousarray(s["psi_sp"][: 2 * mtxd].T)).to("cuda")
+    ht = torch.zeros_like(pt)
+    out = {"spinor_bands": 64, "nsp": 4}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        for _ in range(3):
+            plan.apply_spin(pt, ht, pid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.apply_spin(pt, ht, pid)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        out[prec] = {"ms_per_call": round(ms, 4), "spinor_bands_per_s": round(64 / (ms * 1e-3), 1)}
+    plan.close()
+    plan = hk.Plan.from_inputs(d)
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:mtxd].T)).to("cuda")
+    occ = np.full(psi.shape[0], 2.0)
+    for _ in range(3):
+        plan.density(psi, occ)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.density(psi, occ)
+    torch.cuda.synchronize()
+    out["density_128_bands_ms"] = round((time.perf_counter() - t0) / steps * 1e3, 4)
+    plan.close()
+    return out
+
+
 def bench_cfg2(torch, hk, synth, np, steps):
     """216-atom Si (108^3 mesh, 864 projectors, 512 bands): throughput of both
     paths and the FP32-mixed error against FP64 (H psi norm and Ritz values of
@@ -446,6 +484,8 @@ def main():
     if not args.no_kpoints:
         log("bench: 16 k-points, eager loop vs one graph launch")
         out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))
+    log("bench: two-component bands and the density")
+    out["spinor_density"] = bench_spin_density(torch, hk, synth, np, d, max(5, args.steps // 3))
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
