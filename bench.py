@@ -99,18 +99,21 @@ def call_roofline(info, nanl, nb, prec, ms):
             "roofline_bands_per_s": round(nb / ((t_loc + t_nl) * 1e-6), 1)}
 
 
-def measured_traffic(prec, plane_variant, n1):
-    """HBM bytes per launch of the plane-stage kernel measured with rocprofv3
-    FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on this
-    workload, recorded in profiles/roofline_traffic.json; None if that file has
-    no entry for this kernel."""
+def measured_traffic(prec, kernel, variant, n1):
+    """HBM bytes per launch of a plane-stage or projector kernel measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on
+    this workload, recorded in profiles/roofline_traffic.json under
+    "<prec>/<plane|proj|back><variant>/<n1>"; None if that file has no entry."""
+    fam = {"k_plane": "plane", "k_proj": "proj", "k_back": "back"}.get(kernel)
+    if fam is None:
+        return None, None
     path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None, None
-    e = rec.get("kernels", {}).get(f"{prec}/plane{plane_variant}/{n1}")
+    e = rec.get("kernels", {}).get(f"{prec}/{fam}{variant}/{n1}")
     if not e:
         return None, None
     return e["bytes"], f"rocprofv3 FETCH_SIZE+WRITE_SIZE of {e['kernel']} (profiles/roofline_traffic.json)"
@@ -357,6 +360,42 @@ def bench_cfg2(torch, hk, synth, np, steps):
     return out
 
 
+def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
+    """Standalone (one stream) duration of every projection and back-projection
+    kernel built, both precisions, by HIP events around each launch; then the
+    512 MiB calibration copy.  For profiling these kernels one by one."""
+    hk.set_overlap(False)
+    h = torch.zeros_like(psi)
+    out = {}
+    for prec, pid in (("fp32", hk.PREC_FP32), ("fp64", hk.PREC_FP64)):
+        work = kernel_work(info, nanl, prec)
+        for what, fam, nv in ((hk.TUNE_PROJ, "proj", 5), (hk.TUNE_BACK, "back", 4)):
+            for v in range(nv):
+                plan.set_variant(pid, what, v)
+                for _ in range(2):
+                    plan.apply(psi, h, pid)
+                torch.cuda.synchronize()
+                hk.profile(True, ["k_" + fam])
+                hk.profile_read()
+                for _ in range(reps):
+                    plan.apply(psi, h, pid)
+                torch.cuda.synchronize()
+                ms, cnt = hk.profile_read()["k_" + fam]
+                hk.profile(False)
+                e = roofline_entry("k_" + fam, work, ms / max(cnt, 1), nb, prec)
+                out[f"{prec}/{fam}{v}"] = {"us": round(1e3 * ms / max(cnt, 1), 1), "tflops": e["achieved"],
+                                           "frac": e["frac"]}
+                log(f"bench: {prec} {fam} variant {v}: {out[f'{prec}/{fam}{v}']}")
+            plan.set_variant(pid, what, -1)
+    hk.set_overlap(True)
+    x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    x.fill_(1.0)
+    y = x.clone()  # 512 MiB read + 512 MiB written
+    torch.cuda.synchronize()
+    del x, y
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,6 +408,9 @@ def main():
                     help="skip the iterative diagonalisation section")
     ap.add_argument("--extra", action="store_true",
                     help="also measure the 216-atom cell (108^3 mesh, 512 bands)")
+    ap.add_argument("--variant-sweep", action="store_true",
+                    help="only time every projection / back-projection kernel alone (one "
+                         "stream), then the calibration copy; prints their JSON and exits")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed loops, one plain 512 MiB device copy (a known-size "
                          "dispatch that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -395,6 +437,10 @@ def main():
     psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:mtxd].T)).to("cuda")
     h64 = torch.zeros_like(psi)
     h32 = torch.zeros_like(psi)
+    if args.variant_sweep:
+        print(json.dumps({"variant_sweep": variant_sweep(torch, hk, plan, psi, info, nanl, nb, 10)}), flush=True)
+        plan.close()
+        return
 
     # autotuner: plane-stage kernel and split-K target per precision, timed on
     # this workload (part of the warm-up; every later apply uses the choice)
@@ -493,8 +539,10 @@ def main():
         log("bench: 216-atom cell (108^3, 512 bands)")
         out["si216"] = bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))
     for key, prec in (("roofline", "fp32"), ("roofline_fp64", "fp64")):
-        if out[key]["kernel"] == "k_plane":
-            tb, src = measured_traffic(prec, tuned[prec]["plane"], info["n1"])
+        kern = out[key]["kernel"]
+        fam = {"k_plane": "plane", "k_proj": "proj", "k_back": "back"}.get(kern)
+        if fam is not None:
+            tb, src = measured_traffic(prec, kern, tuned[prec][fam], info["n1"])
             out[key]["traffic"] = tb
             out[key]["traffic_source"] = src
     # the Fortran-facing drop-in: host arrays in and out (PCIe included)

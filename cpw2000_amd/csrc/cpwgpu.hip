@@ -193,6 +193,10 @@ struct cpwgpu_plan_s {
     // back-projection GEMM kernel per precision: 0 LDS-staged k_back, 1
     // k_back_direct (-1 = default)
     int back_var[2] = {-1, -1};
+    // where the kinetic term ekpg*psi is added when the non-local term is on:
+    // 0 the back-projection epilogue, 1 the gather of the forward column kernel
+    // (-1 = default, 0)
+    int kin_var[2] = {-1, -1};
     // workspaces
     DevBuf colbuf, part, dhd;
     DevBuf gw;                // g_j of each plane wave as doubles [3][mtxd] (force projections)
@@ -689,7 +693,7 @@ int ensure_aux() {
 // hpsi; join (may be null): event the first gather waits for (that term).
 template <typename R>
 int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-               bool acc, hipEvent_t join, hipStream_t s) {
+               bool acc, hipEvent_t join, hipStream_t s, bool kin_gather = false) {
     const FftGeom &g = p->g;
     const size_t cs = sizeof(cplx<R>);
     const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
@@ -705,7 +709,8 @@ int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
         // with the non-local term the back-projection already wrote it and the
         // kinetic term into hpsi; without it the gather adds the kinetic term
         const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
-                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, acc ? COLF_ACC : COLF_KIN};
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb,
+                           acc ? (COLF_ACC | (kin_gather ? COLF_KIN : 0)) : COLF_KIN};
         if (g.ncol > 0) {
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, s))) return st;
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, s))) return st;
@@ -724,20 +729,22 @@ template <typename R>
 int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
             hipStream_t s) {
     const bool nl = p->nanl > 0 && p->g.mtxd > 0;
+    // kinetic term in the gather (kg) or in the back-projection epilogue
+    const bool kg = p->kin_var[sizeof(R) == 4 ? 1 : 0] == 1;
     int st;
     if (nl && !g_overlap) {
         // serial: non-local term first (writes hpsi), then the local term adds
-        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, true, s))) return st;
-        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s);
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, s))) return st;
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s, kg);
     }
     if (nl) {
         // fork: non-local term on the second stream, written into hpsi
         if ((st = ensure_aux())) return st;
         HIPCHK(hipEventRecord(g_fork, s));
         HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
-        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, true, g_aux))) return st;
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, g_aux))) return st;
         HIPCHK(hipEventRecord(g_join, g_aux));
-        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s);
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s, kg);
     }
     return local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s);
 }
@@ -1714,21 +1721,27 @@ struct TuneKey {
     }
 };
 
-bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk) {
+// one line per key: prec n1 n2 n3 nx ncol nanl neig | plane proj back splitk kin
+// (lines without the last field, kin, read it as -1)
+bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk, int *kin) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return false;
     FILE *f = std::fopen(path, "r");
     if (!f) return false;
-    TuneKey r;
-    int pv, jv, bv, sk;
+    char line[256];
     bool found = false;
-    while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3, &r.nx,
-                       &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk) == 12) {
+    while (std::fgets(line, sizeof line, f)) {
+        TuneKey r;
+        int pv, jv, bv, sk, kv = -1;
+        const int n = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3,
+                                  &r.nx, &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk, &kv);
+        if (n < 12) continue;
         if (r == k) {  // the last entry for a key wins
             *plane = pv;
             *proj = jv;
             *back = bv;
             *splitk = sk;
+            *kin = n == 13 ? kv : -1;
             found = true;
         }
     }
@@ -1736,13 +1749,13 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
     return found;
 }
 
-void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk) {
+void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk, int kin) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
-                 k.nanl, k.neig, plane, proj, back, splitk);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
+                 k.nanl, k.neig, plane, proj, back, splitk, kin);
     std::fclose(f);
 }
 
@@ -1773,8 +1786,8 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
     const FftGeom &g = p->g;
     const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
-    int pv = -1, jv = -1, bv = -1, sk = -1;
-    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk)) {
+    int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1;
+    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv)) {
         bool ok;
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
@@ -1782,6 +1795,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         p->proj_var[ix] = jv >= 0 && jv <= NPROJ_DIRECT ? jv : -1;
         p->back_var[ix] = bv >= 0 && bv <= NBACK_DIRECT ? bv : -1;
         p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
+        p->kin_var[ix] = kv == 0 || kv == 1 ? kv : -1;
         return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
     }
     int st;
@@ -1822,9 +1836,20 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         }
     }
     p->back_var[ix] = best_bv;
+    // where the kinetic term goes: it moves work between the two streams
+    int best_kv = 0;
+    if (p->nanl > 0) {
+        best = 3.4e38f;
+        for (int kv2 = 0; kv2 <= 1; kv2++) {
+            p->kin_var[ix] = kv2;
+            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+            if (t < best) { best = t; best_kv = kv2; }
+        }
+    }
+    p->kin_var[ix] = best_kv;
     // leave hpsi holding H psi of the chosen configuration
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
-    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk);
+    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv);
     return CPWGPU_OK;
 }
 
@@ -2138,6 +2163,11 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         plan->back_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
+    if (what == CPWGPU_TUNE_KIN) {
+        if (value > 1) return fail(CPWGPU_ERR_MODE, "kinetic-term placement %d", value);
+        plan->kin_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
     if (what == CPWGPU_TUNE_SPLITK) {
         if (value >= 0 && (value < 64 || value > 4096))
             return fail(CPWGPU_ERR_ARG, "split-K target %d outside 64..4096", value);
@@ -2167,6 +2197,10 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
     }
     if (what == CPWGPU_TUNE_BACK) {
         *value = plan->back_var[ix] >= 1 && plan->back_var[ix] <= NBACK_DIRECT ? plan->back_var[ix] : 0;
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_KIN) {
+        *value = plan->kin_var[ix] == 1 ? 1 : 0;
         return CPWGPU_OK;
     }
     return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);

@@ -54,7 +54,7 @@ EXPORTS = (
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
-TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK = 0, 1, 2, 3
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN = 0, 1, 2, 3, 4
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
@@ -342,7 +342,7 @@ class Plan:
 
     def tune(self, psi_t, hpsi_t, prec=PREC_FP32, stream=None):
         """Autotune the plan's kernels on these device arrays (hpsi_t receives
-        H psi); returns {"plane": variant, "splitk": target}."""
+        H psi); returns the choice {"plane", "proj", "back", "splitk", "kin"}."""
         import torch
 
         neig = psi_t.shape[0]
@@ -354,7 +354,8 @@ class Plan:
         return {"plane": self.get_variant(prec, TUNE_PLANE),
                 "proj": self.get_variant(prec, TUNE_PROJ),
                 "back": self.get_variant(prec, TUNE_BACK),
-                "splitk": self.get_variant(prec, TUNE_SPLITK)}
+                "splitk": self.get_variant(prec, TUNE_SPLITK),
+                "kin": self.get_variant(prec, TUNE_KIN)}
 
     def set_potential(self, vscr):
         v = np.ascontiguousarray(vscr, np.float64)

@@ -157,11 +157,16 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *                       -1 = default (0)
  *   CPWGPU_TUNE_BACK    kernel of the back-projection GEMM: 0 = LDS-staged D,
  *                       1..3 = direct register loads in three tile shapes;
- *                       -1 = default (0) */
+ *                       -1 = default (0)
+ *   CPWGPU_TUNE_KIN     where ekpg*psi is added when projectors are set: 0 = in
+ *                       the back-projection epilogue, 1 = in the gather of the
+ *                       forward column kernel (moves work between the two
+ *                       streams); -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2
 #define CPWGPU_TUNE_BACK 3
+#define CPWGPU_TUNE_KIN 4
 int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
 /* Current choice (the value used by the next apply, defaults resolved). */
 int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);

@@ -78,6 +78,7 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
             for jv in (0, 1):
                 plan.set_variant(prec, hk.TUNE_PROJ, jv)
                 plan.set_variant(prec, hk.TUNE_BACK, min(jv, 3))
+                plan.set_variant(prec, hk.TUNE_KIN, jv)  # kinetic term in the epilogue / the gather
                 h = _apply(torch, plan, pt, prec).cpu().numpy().T
                 assert rel(h, href) < tol, (name, prec, v, jv, rel(h, href))
     plan.close()
@@ -101,6 +102,12 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "back", bv)
         plan.set_variant(prec, hk.TUNE_BACK, -1)
+        for kv in (1, 0):
+            plan.set_variant(prec, hk.TUNE_KIN, kv)
+            assert plan.get_variant(prec, hk.TUNE_KIN) == kv
+            h = _apply(torch, plan, pt, prec)
+            assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "kin", kv)
+        plan.set_variant(prec, hk.TUNE_KIN, -1)
         for jv in (0, 1, 2, 3, 4):
             plan.set_variant(prec, hk.TUNE_PROJ, jv)
             for sk in (64, 512, 4096):
@@ -117,7 +124,7 @@ def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 5), (hk.TUNE_BACK, 4), (7, 0)):
+                        (hk.TUNE_PROJ, 5), (hk.TUNE_BACK, 4), (hk.TUNE_KIN, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -141,10 +148,12 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         assert choice["plane"] in _variants(hk, plan, prec)
         assert choice["proj"] in (0, 1, 2, 3, 4) and choice["back"] in (0, 1, 2, 3)
         assert choice["splitk"] in (128, 256, 512, 1024)
+        assert choice["kin"] in (0, 1)
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
         plan.set_variant(prec, hk.TUNE_PROJ, choice["proj"])
         plan.set_variant(prec, hk.TUNE_BACK, choice["back"])
         plan.set_variant(prec, hk.TUNE_SPLITK, choice["splitk"])
+        plan.set_variant(prec, hk.TUNE_KIN, choice["kin"])
         # the tuner leaves H psi of its choice in hpsi
         assert float(torch.linalg.norm(ht - ref) / torch.linalg.norm(ref)) < tol
     lines = cache.read_text().split("\n")
@@ -156,7 +165,8 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
     assert c2 == {"plane": plan.get_variant(hk.PREC_FP32, hk.TUNE_PLANE),
                   "proj": plan.get_variant(hk.PREC_FP32, hk.TUNE_PROJ),
                   "back": plan.get_variant(hk.PREC_FP32, hk.TUNE_BACK),
-                  "splitk": plan.get_variant(hk.PREC_FP32, hk.TUNE_SPLITK)}
+                  "splitk": plan.get_variant(hk.PREC_FP32, hk.TUNE_SPLITK),
+                  "kin": plan.get_variant(hk.PREC_FP32, hk.TUNE_KIN)}
     assert len([l for l in cache.read_text().split("\n") if l.strip()]) == 2
     plan.close()
     plan2.close()
