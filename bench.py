@@ -369,7 +369,7 @@ def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
     out = {}
     for prec, pid in (("fp32", hk.PREC_FP32), ("fp64", hk.PREC_FP64)):
         work = kernel_work(info, nanl, prec)
-        for what, fam, nv in ((hk.TUNE_PROJ, "proj", 5), (hk.TUNE_BACK, "back", 4)):
+        for what, fam, nv in ((hk.TUNE_PROJ, "proj", 6), (hk.TUNE_BACK, "back", 5)):
             for v in range(nv):
                 plan.set_variant(pid, what, v)
                 for _ in range(2):

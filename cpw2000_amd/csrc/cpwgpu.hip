@@ -417,6 +417,12 @@ template <> struct RegShape<double> {
     static constexpr int T[3] = {256, 320, 256}, W[3] = {3, 4, 3};
 };
 constexpr int NPLANE_VAR = 4;
+// back-projection kernels: 0 LDS-staged, 1..NBACK_DIRECT direct, NBACK_DIRECT+1 LDS-staged with half chunks
+constexpr int NBACK_VAR = NBACK_DIRECT + 2;
+// projection kernels: 0 LDS-staged, 1..NPROJ_DIRECT direct, NPROJ_DIRECT+1 LDS-staged with half wave tiles
+constexpr int NPROJ_VAR = NPROJ_DIRECT + 2;
+template <typename R> constexpr int proj_half_wm() { return GemmCfg<R>::WM / 2; }
+template <typename R> constexpr int proj_half_wn() { return GemmCfg<R>::WN > 1 ? GemmCfg<R>::WN / 2 : 1; }
 
 // a register shape is built for a mesh length when both splits divide N and a
 // lane holds at most 288 bytes of one line
@@ -610,9 +616,13 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
     }
     const long lda = sizeof(R) == 8 ? pv_.lda : *pv_.lda32;
     const int ix = sizeof(R) == 4 ? 1 : 0;
-    const int pv = p->proj_var[ix] >= 1 && p->proj_var[ix] <= NPROJ_DIRECT ? p->proj_var[ix] : 0;
+    const int pv = p->proj_var[ix] >= 1 && p->proj_var[ix] < NPROJ_VAR ? p->proj_var[ix] : 0;
     // block tile and K granule of the chosen projection kernel
     int BM = proj_bm<R>(), BN = proj_bn<R>(), BK = GemmCfg<R>::BK;
+    if (pv == NPROJ_DIRECT + 1) {
+        BM = 2 * proj_half_wm<R>() * 16;
+        BN = 2 * proj_half_wn<R>() * 16;
+    }
 #define CPW_PD(V) if (pv == V) { BM = direct_bm<R, V>(); BN = direct_bn<R, V>(); BK = direct_bk<R, V>(); }
     CPW_PD(1) CPW_PD(2) CPW_PD(3) CPW_PD(4)
 #undef CPW_PD
@@ -638,6 +648,10 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
             hipLaunchKernelGGL((k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else if (pv == 4)
             hipLaunchKernelGGL((k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
+        else if (pv == NPROJ_DIRECT + 1)
+            hipLaunchKernelGGL((k_proj<R, proj_half_wm<R>(), proj_half_wn<R>()>), dim3(tm, tn, nsl),
+                               dim3(GEMM_THREADS), 0, s, anl, lda, (const double2 *)psi, ldp,
+                               (cplx<R> *)p->part.p, rows, nanl, neig, kslice, nullptr);
         else
             hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
                                (const double2 *)psi, ldp, (cplx<R> *)p->part.p, rows, nanl, neig,
@@ -650,7 +664,7 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
     }
     {
         ProfScope ps_(CPWGPU_K_BACK, s);
-        const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_DIRECT ? p->back_var[ix] : 0;
+        const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_VAR - 1 ? p->back_var[ix] : 0;
         const cplx<R> *dh = (const cplx<R> *)p->dhd.p;
         double2 *hp = (double2 *)hpsi;
         // the kinetic term ekpg * psi is added here (coalesced), not in the gather
@@ -667,6 +681,11 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
 #undef CPW_BD
         if (bv == 0)
             hipLaunchKernelGGL(k_back<R>, dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+                               dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta,
+                               kp, ldp, ek);
+        else if (bv == NBACK_DIRECT + 1)
+            hipLaunchKernelGGL((k_back<R, GemmCfg<R>::KC / 2>),
+                               dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
                                dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta,
                                kp, ldp, ek);
     }
@@ -1792,8 +1811,8 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
         p->plane_var[ix] = ok ? pv : -1;
-        p->proj_var[ix] = jv >= 0 && jv <= NPROJ_DIRECT ? jv : -1;
-        p->back_var[ix] = bv >= 0 && bv <= NBACK_DIRECT ? bv : -1;
+        p->proj_var[ix] = jv >= 0 && jv < NPROJ_VAR ? jv : -1;
+        p->back_var[ix] = bv >= 0 && bv < NBACK_VAR ? bv : -1;
         p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
         p->kin_var[ix] = kv == 0 || kv == 1 ? kv : -1;
         return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
@@ -1816,7 +1835,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         // projection kernel x split-K target
         const int cand[4] = {128, 256, 512, 1024};
         best = 3.4e38f;
-        for (int jv = 0; jv <= NPROJ_DIRECT; jv++)
+        for (int jv = 0; jv < NPROJ_VAR; jv++)
             for (int c : cand) {
                 p->proj_var[ix] = jv;
                 p->splitk[ix] = c;
@@ -1829,7 +1848,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     int best_bv = 0;
     if (p->nanl > 0) {
         best = 3.4e38f;
-        for (int bv = 0; bv <= NBACK_DIRECT; bv++) {
+        for (int bv = 0; bv < NBACK_VAR; bv++) {
             p->back_var[ix] = bv;
             if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
             if (t < best) { best = t; best_bv = bv; }
@@ -2154,12 +2173,12 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_PROJ) {
-        if (value > NPROJ_DIRECT) return fail(CPWGPU_ERR_MODE, "projection kernel %d", value);
+        if (value >= NPROJ_VAR) return fail(CPWGPU_ERR_MODE, "projection kernel %d", value);
         plan->proj_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_BACK) {
-        if (value > NBACK_DIRECT) return fail(CPWGPU_ERR_MODE, "back-projection kernel %d", value);
+        if (value >= NBACK_VAR) return fail(CPWGPU_ERR_MODE, "back-projection kernel %d", value);
         plan->back_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
@@ -2192,11 +2211,11 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_PROJ) {
-        *value = plan->proj_var[ix] >= 1 && plan->proj_var[ix] <= NPROJ_DIRECT ? plan->proj_var[ix] : 0;
+        *value = plan->proj_var[ix] >= 1 && plan->proj_var[ix] < NPROJ_VAR ? plan->proj_var[ix] : 0;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_BACK) {
-        *value = plan->back_var[ix] >= 1 && plan->back_var[ix] <= NBACK_DIRECT ? plan->back_var[ix] : 0;
+        *value = plan->back_var[ix] >= 1 && plan->back_var[ix] < NBACK_VAR ? plan->back_var[ix] : 0;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_KIN) {

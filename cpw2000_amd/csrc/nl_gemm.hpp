@@ -74,7 +74,9 @@ template <typename R> constexpr int proj_bn() { return 2 * GemmCfg<R>::WN * 16; 
 // twice).  Double-buffered LDS: one barrier per K stage, the next stage's
 // global loads in flight during the current stage's MFMAs.
 // ---------------------------------------------------------------------------
-template <typename R>
+// WMX x WNX: MFMA tiles per wave (variant 0 uses GemmCfg<R>; half of it in
+// each direction keeps fewer accumulators per lane, for more waves per SIMD)
+template <typename R, int WMX = GemmCfg<R>::WM, int WNX = GemmCfg<R>::WN>
 __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict__ anl, long lda,
                                                        const double2 *__restrict__ psi, long ldp,
                                                        cplx<R> *__restrict__ part, int mtxd,
@@ -82,9 +84,10 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
                                                        const double *__restrict__ wrow = nullptr,
                                                        int lower = 0) {
     typedef typename MF<R>::v4 v4;
-    constexpr int WM = GemmCfg<R>::WM, WN = GemmCfg<R>::WN;
+    constexpr int WM = WMX, WN = WNX;
     constexpr int BK = GemmCfg<R>::BK, KP = GemmCfg<R>::KP;
-    constexpr int BM = proj_bm<R>(), BN = proj_bn<R>();
+    constexpr int BM = 2 * WM * 16, BN = 2 * WN * 16;
+    static_assert(BM * BK % GEMM_THREADS == 0 && BN * BK % GEMM_THREADS == 0, "bad k_proj tile");
     constexpr int KPL = BK / 4;                   // K values per lane group per stage
     constexpr int LPA = BM * BK / GEMM_THREADS;   // complex values a thread stages (A)
     constexpr int LPB = BN * BK / GEMM_THREADS;   // (psi)
@@ -439,7 +442,9 @@ constexpr int BP_BN = 64;  // bands per block
 // as three planes (re, im, re+im); projector values and the next dhd chunk
 // are loaded into registers while the current chunk is multiplied.
 // ---------------------------------------------------------------------------
-template <typename R>
+// KCX: projectors per LDS chunk (variant 0 uses GemmCfg<R>::KC; half of it
+// keeps fewer values in flight per lane, for more waves per SIMD)
+template <typename R, int KCX = GemmCfg<R>::KC>
 __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict__ anl, long lda,
                                                        const cplx<R> *__restrict__ dhd,
                                                        double2 *__restrict__ hpsi, long ldh,
@@ -448,7 +453,9 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
                                                        long ldk = 0,
                                                        const double *__restrict__ ekpg = nullptr) {
     typedef typename MF<R>::v4 v4;
-    constexpr int KC = GemmCfg<R>::KC, KCP = GemmCfg<R>::KCP;
+    // LDS rows padded by 16 bytes: an odd number of 16-byte units per row
+    constexpr int KC = KCX, KCP = KCX + 16 / (int)sizeof(R);
+    static_assert((KCP * sizeof(R) / 16) % 2 == 1 && KC % 16 == 0, "bad k_back chunk");
     constexpr int KPL = KC / 4;             // projectors per lane group per chunk
     constexpr int DPT = BP_BN * KC / GEMM_THREADS;  // dhd values a thread stages
     __shared__ __align__(16) R sD[3][BP_BN][KCP];   // re, im, re+im

@@ -153,10 +153,12 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
  *                       GEMM (64..4096); -1 = default (256)
  *   CPWGPU_TUNE_PROJ    kernel of the projection GEMM: 0 = LDS-staged tiles,
- *                       1..4 = direct register loads in four tile shapes;
+ *                       1..4 = direct register loads in four tile shapes,
+ *                       5 = LDS-staged with half the wave tile;
  *                       -1 = default (0)
  *   CPWGPU_TUNE_BACK    kernel of the back-projection GEMM: 0 = LDS-staged D,
- *                       1..3 = direct register loads in three tile shapes;
+ *                       1..3 = direct register loads in three tile shapes,
+ *                       4 = LDS-staged with half-size projector chunks;
  *                       -1 = default (0)
  *   CPWGPU_TUNE_KIN     where ekpg*psi is added when projectors are set: 0 = in
  *                       the back-projection epilogue, 1 = in the gather of the

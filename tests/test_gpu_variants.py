@@ -97,7 +97,7 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             plan.set_variant(prec, hk.TUNE_PLANE, v)
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, v)
-        for bv in (0, 1, 2, 3):
+        for bv in (0, 1, 2, 3, 4):
             plan.set_variant(prec, hk.TUNE_BACK, bv)
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "back", bv)
@@ -108,7 +108,7 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "kin", kv)
         plan.set_variant(prec, hk.TUNE_KIN, -1)
-        for jv in (0, 1, 2, 3, 4):
+        for jv in (0, 1, 2, 3, 4, 5):
             plan.set_variant(prec, hk.TUNE_PROJ, jv)
             for sk in (64, 512, 4096):
                 plan.set_variant(prec, hk.TUNE_SPLITK, sk)
@@ -124,7 +124,7 @@ def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 5), (hk.TUNE_BACK, 4), (hk.TUNE_KIN, 2), (7, 0)):
+                        (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -146,7 +146,7 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         choice = plan.tune(pt, ht, prec)
         torch.cuda.synchronize()
         assert choice["plane"] in _variants(hk, plan, prec)
-        assert choice["proj"] in (0, 1, 2, 3, 4) and choice["back"] in (0, 1, 2, 3)
+        assert choice["proj"] in range(6) and choice["back"] in range(5)
         assert choice["splitk"] in (128, 256, 512, 1024)
         assert choice["kin"] in (0, 1)
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
