@@ -99,6 +99,22 @@ def call_roofline(info, nanl, nb, prec, ms):
             "roofline_bands_per_s": round(nb / ((t_loc + t_nl) * 1e-6), 1)}
 
 
+def local_roofline(info, nb, prec, per):
+    """The local stage alone (z-column inverse, plane stage, z-column forward
+    and gather, one stream) against its HBM bound: the same per-band bytes as
+    the local part of call_roofline; measured = sum of the three kernels'
+    average HIP-event durations (ms) in `per`."""
+    s_c = 16 if prec == "fp64" else 8
+    s_r = 8 if prec == "fp64" else 4
+    n = info["n1"] * info["n2"] * info["n3"]
+    b_loc = s_c * 2 * info["mtxd"] + 4 * s_c * info["ncol"] * info["n3"] + s_r * n
+    t_roof = b_loc * nb / (PEAK_HBM_GBS * 1e9) * 1e6
+    parts = {k: round(per[k] * 1e3, 1) for k in ("k_col_bwd", "k_plane", "k_col_fwd")}
+    t_meas = sum(parts.values())
+    return {"t_roof_us": round(t_roof, 1), "t_measured_us": round(t_meas, 1), "kernels_us": parts,
+            "achieved_gbs": round(b_loc * nb / (t_meas * 1e-6) / 1e9, 1), "frac": round(t_roof / t_meas, 4)}
+
+
 def measured_traffic(prec, kernel, variant, n1):
     """HBM bytes per launch of a plane-stage or projector kernel measured with
     rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on
@@ -360,18 +376,77 @@ def bench_cfg2(torch, hk, synth, np, steps):
     return out
 
 
+def issue_bound(prec, kernel, variant, n1, avg_ms, rates):
+    """Instruction-issue floor of a plane-stage kernel: its wave-level VALU
+    instructions per launch by kind (rocprofv3 SQ_INSTS_VALU_* counters,
+    profiles/roofline_traffic.json "valu") times the issue cost of each kind
+    measured on this GPU (cycles per wave instruction per SIMD), over all SIMDs
+    at the measured shader clock; frac = floor / measured duration."""
+    if kernel != "k_plane" or not rates or "v_add_f32" not in rates:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "roofline_traffic.json")) as f:
+            e = json.load(f).get("valu", {}).get(f"{prec}/plane{variant}/{n1}")
+    except (OSError, ValueError):
+        return None
+    if not e:
+        return None
+    # DPP lane exchanges (counted by the hardware as FP32 adds or as moves,
+    # "dpp_in") issue at the DPP rate; the remaining 32-bit work at the plain rates
+    dpp = e.get("dpp", 0)
+    f32 = e["f32"] - (dpp if e.get("dpp_in") == "f32" else 0)
+    other = e["other"] - (dpp if e.get("dpp_in") == "other" else 0)
+    r64 = 0.5 * (rates["v_fma_f64"] + rates["v_add_f64"])
+    cyc = (e["f64"] * r64 + dpp * rates["v_mov_b32_dpp"] + f32 * rates["v_fma_f32"]
+           + e["int32"] * rates["v_add_u32"] + other * rates["v_add_f32"]) / rates["simds"]
+    floor_us = cyc / (rates["shader_clock_ghz"] * 1e3)
+    return {"kernel": e["kernel"], "valu_instr": e["valu"], "dpp_instr": dpp, "cycles_per_simd": round(cyc),
+            "clock_ghz": rates["shader_clock_ghz"], "floor_us": round(floor_us, 1),
+            "measured_us": round(avg_ms * 1e3, 1), "frac": round(floor_us / (avg_ms * 1e3), 4),
+            "source": "SQ_INSTS_VALU_* of profiles/roofline_traffic.json x bench.py --issue-rates"}
+
+
+ISSUE_KINDS = ("v_fma_f64", "v_add_f64", "v_fma_f32", "v_add_u32", "v_mov_b32_dpp", "v_fma_f64+v_mov_b32_dpp",
+               "v_add_f32", "v_add_f32_dpp")
+
+
+def issue_rates():
+    """Cycles per wave instruction per SIMD of the VALU instruction kinds the
+    FFT kernels are made of, measured on this GPU (libcpwprobe.so), and the
+    shader clock during the measurement."""
+    import ctypes as C
+
+    import torch  # noqa: F401  (one HIP runtime in the process: torch's)
+
+    lib = C.CDLL(os.path.join(ROOT, "cpw2000_amd", "libcpwprobe.so"))
+    nk = len(ISSUE_KINDS)
+    res = (C.c_double * (nk + 2))()
+    st = lib.cpwprobe_issue_rates(res, nk + 2)
+    if st != 0:
+        raise RuntimeError(f"cpwprobe_issue_rates: HIP error {st}")
+    out = {k: round(res[i], 3) for i, k in enumerate(ISSUE_KINDS)}
+    out["shader_clock_ghz"] = round(res[nk], 3)
+    out["simds"] = int(res[nk + 1])
+    return out
+
+
 def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
-    """Standalone (one stream) duration of every projection and back-projection
-    kernel built, both precisions, by HIP events around each launch; then the
-    512 MiB calibration copy.  For profiling these kernels one by one."""
+    """Standalone (one stream) duration of every plane-stage, projection and
+    back-projection kernel built, both precisions, by HIP events around each
+    launch; then the 512 MiB calibration copy.  For profiling these kernels one
+    by one."""
     hk.set_overlap(False)
     h = torch.zeros_like(psi)
     out = {}
     for prec, pid in (("fp32", hk.PREC_FP32), ("fp64", hk.PREC_FP64)):
         work = kernel_work(info, nanl, prec)
-        for what, fam, nv in ((hk.TUNE_PROJ, "proj", 6), (hk.TUNE_BACK, "back", 5)):
+        for what, fam, nv in ((hk.TUNE_PLANE, "plane", 4), (hk.TUNE_PROJ, "proj", 6),
+                              (hk.TUNE_BACK, "back", 5)):
             for v in range(nv):
-                plan.set_variant(pid, what, v)
+                try:
+                    plan.set_variant(pid, what, v)
+                except hk.CpwGpuError:
+                    continue  # plane shape not built for this mesh
                 for _ in range(2):
                     plan.apply(psi, h, pid)
                 torch.cuda.synchronize()
@@ -383,8 +458,8 @@ def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
                 ms, cnt = hk.profile_read()["k_" + fam]
                 hk.profile(False)
                 e = roofline_entry("k_" + fam, work, ms / max(cnt, 1), nb, prec)
-                out[f"{prec}/{fam}{v}"] = {"us": round(1e3 * ms / max(cnt, 1), 1), "tflops": e["achieved"],
-                                           "frac": e["frac"]}
+                out[f"{prec}/{fam}{v}"] = {"us": round(1e3 * ms / max(cnt, 1), 1), "achieved": e["achieved"],
+                                           "unit": e["unit"], "frac": e["frac"]}
                 log(f"bench: {prec} {fam} variant {v}: {out[f'{prec}/{fam}{v}']}")
             plan.set_variant(pid, what, -1)
     hk.set_overlap(True)
@@ -411,6 +486,9 @@ def main():
     ap.add_argument("--variant-sweep", action="store_true",
                     help="only time every projection / back-projection kernel alone (one "
                          "stream), then the calibration copy; prints their JSON and exits")
+    ap.add_argument("--issue-rates", action="store_true",
+                    help="only measure the issue cost of the FFT kernels' VALU instruction kinds "
+                         "(cycles per wave instruction per SIMD) and the shader clock; prints JSON")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed loops, one plain 512 MiB device copy (a known-size "
                          "dispatch that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -429,6 +507,9 @@ def main():
     from cpw2000_amd import synth
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.issue_rates:
+        print(json.dumps({"issue_rates": issue_rates()}), flush=True)
+        return
     log("bench: generating 64-atom Si workload")
     d = synth.make_config(1)
     plan = hk.Plan.from_inputs(d)
@@ -473,6 +554,15 @@ def main():
     for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
         _, cal[prec] = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 3, 1, None)
     work = {p: kernel_work(info, nanl, p) for p in ("fp64", "fp32")}
+    # the local stage on one stream (no projector GEMMs beside it)
+    hk.set_overlap(False)
+    local = {}
+    for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        _, per = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 5, 1,
+                           ["k_col_bwd", "k_plane", "k_col_fwd"])
+        local[prec] = local_roofline(info, nb, prec, per)
+    hk.set_overlap(True)
+    log(f"bench: local stage alone {local}")
 
     def dominant(prec):
         return max((k for k in cal[prec] if k in work[prec]), key=lambda k: cal[prec][k])
@@ -516,6 +606,7 @@ def main():
         "roofline_fp64": roofline_entry(res["fp64"]["dominant"], work["fp64"],
                                         res["fp64"]["dom_ms"], nb, "fp64"),
         "call_roofline": {p: call_roofline(info, nanl, nb, p, res[p]["ms"]) for p in ("fp32", "fp64")},
+        "local_roofline": local,
         "tuned": tuned,
         "kernel_ms": kernels,
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
@@ -545,6 +636,22 @@ def main():
             tb, src = measured_traffic(prec, kern, tuned[prec][fam], info["n1"])
             out[key]["traffic"] = tb
             out[key]["traffic_source"] = src
+    # the plane stage's real bound: VALU instruction issue (cost per kind measured here)
+    try:
+        rates = issue_rates()
+    except (OSError, RuntimeError) as exc:
+        log(f"bench: issue-rate probe unavailable ({exc})")
+        rates = None
+    out["issue_rates"] = rates
+    for key, prec in (("roofline", "fp32"), ("roofline_fp64", "fp64")):
+        iss = issue_bound(prec, out[key]["kernel"], tuned[prec]["plane"], info["n1"],
+                          out[key]["avg_ms"], rates)
+        if iss is not None:
+            # the same kernel alone on one stream (no projector GEMMs beside it)
+            alone = local[prec]["kernels_us"]["k_plane"]
+            iss["measured_alone_us"] = alone
+            iss["frac_alone"] = round(iss["floor_us"] / alone, 4)
+        out[key]["issue"] = iss
     # the Fortran-facing drop-in: host arrays in and out (PCIe included)
     hk.set_precision(hk.PREC_FP64)
     hpsi_h = np.zeros_like(d["psi"], order="F")

@@ -2,6 +2,7 @@
 
   libcpwgpu.so          HIP kernels + C ABI (include/cpwgpu.h), gfx950
   libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16, hk_psi_spin_c16 and ditsp_c16 over the C ABI (flang)
+  libcpwprobe.so        VALU issue-rate probe (measurement only: ``bench.py --issue-rates``)
 
 Run ``python -m cpw2000_amd.build``.
 """
@@ -18,6 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libcpwgpu.so")
 FLIB = os.path.join(PKG, "libcpwgpu_fortran.so")
+PLIB = os.path.join(PKG, "libcpwprobe.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang")
@@ -46,6 +48,12 @@ def build_hip():
     os.replace(tmp, LIB)
 
 
+def build_probe():
+    tmp = PLIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "issue_rates.hip")])
+    os.replace(tmp, PLIB)
+
+
 def build_fortran():
     srcs = [os.path.join(PKG, "fortran", f)
             for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90")]
@@ -58,6 +66,7 @@ def build_fortran():
 
 def build(fortran: bool = True):
     build_hip()
+    build_probe()
     if fortran:
         build_fortran()
 

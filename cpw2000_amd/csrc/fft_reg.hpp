@@ -78,7 +78,16 @@ __device__ __forceinline__ int opaque(int x) {
 
 template <int H, typename R> __device__ __forceinline__ cplx<R> bfly(cplx<R> m, R s) {
     constexpr int C = dpp_ctrl<H>();
-    return mk<R>(xlane<C>(m.x) + s * m.x, xlane<C>(m.y) + s * m.y);  // partner + s * mine
+    if constexpr (sizeof(R) == 4) {
+        // partner + (s * mine) with the product rounded first (no contraction
+        // into an FMA): the lane move then folds into the add as its DPP
+        // source operand (one v_add_f32 with DPP instead of a move and an FMA)
+#pragma clang fp contract(off)
+        const R tx = s * m.x, ty = s * m.y;
+        return mk<R>(xlane<C>(m.x) + tx, xlane<C>(m.y) + ty);
+    } else {
+        return mk<R>(xlane<C>(m.x) + s * m.x, xlane<C>(m.y) + s * m.y);  // partner + s * mine
+    }
 }
 template <int DIR, typename R> __device__ __forceinline__ cplx<R> cmul_dir(cplx<R> a, cplx<R> w) {
     return cmul(a, DIR > 0 ? w : mk<R>(w.x, -w.y));

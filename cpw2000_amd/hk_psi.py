@@ -330,9 +330,12 @@ class Plan:
     def set_variant(self, prec, what, value):
         """Kernel choice for a precision: what = TUNE_PLANE (plane-stage kernel,
         0 LDS two-pass, 1..3 register shapes, -1 default), TUNE_PROJ
-        (projection GEMM, 0 LDS-staged, 1..4 direct-load shapes), TUNE_BACK
-        (back-projection GEMM, 0 LDS-staged, 1..3 direct-load shapes) or TUNE_SPLITK
-        (projection GEMM split-K target blocks, -1 default)."""
+        (projection GEMM, 0 LDS-staged, 1..4 direct-load shapes, 5 LDS-staged
+        with half-size wave tiles), TUNE_BACK (back-projection GEMM, 0
+        LDS-staged, 1..3 direct-load shapes, 4 LDS-staged with half-size chunks),
+        TUNE_SPLITK (projection GEMM split-K target blocks, -1 default) or
+        TUNE_KIN (kinetic term in the back-projection epilogue 0 or in the
+        column gather 1)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 
     def get_variant(self, prec, what):
