@@ -80,8 +80,9 @@ template <int H, typename R> __device__ __forceinline__ cplx<R> bfly(cplx<R> m, 
     constexpr int C = dpp_ctrl<H>();
     if constexpr (sizeof(R) == 4) {
         // partner + (s * mine) with the product rounded first (no contraction
-        // into an FMA): the lane move then folds into the add as its DPP
-        // source operand (one v_add_f32 with DPP instead of a move and an FMA)
+        // into an FMA; exact, s = +-1): the lane move then folds into the add
+        // as its DPP source operand (a multiply and a DPP-sourced add instead
+        // of a DPP move and an FMA)
 #pragma clang fp contract(off)
         const R tx = s * m.x, ty = s * m.y;
         return mk<R>(xlane<C>(m.x) + tx, xlane<C>(m.y) + ty);
