@@ -16,6 +16,18 @@ def pytest_configure(config):
 
 GOLDEN_CASES = ("cfg0", "cfg0_k", "cfg0_fullmesh", "cfg0_pad", "cfg0_nonl", "ortho", "cfg1_b2")
 
+# the specialised mesh lengths of the device kernels beyond 24 and 72 (the
+# golden cases): (workload, kmscr(1:3) raised so that size_fft gives n, n)
+ENLARGED_MESHES = ((0, 23, 48), (0, 31, 64), (1, 39, 80), (1, 47, 96), (1, 53, 108), (1, 63, 128))
+
+
+def mesh_config(cfg, kmax, nbands):
+    """Workload cfg with the FFT box enlarged to kmscr(1:3) = kmax (same
+    sphere, potential and projectors; more mesh points)."""
+    from cpw2000_amd import synth
+
+    return synth.make_config(cfg, nbands=nbands, kmax_override=[kmax] * 3)
+
 
 def load_golden(name):
     """(inputs dict, psi slice, reference hpsi) of a golden fixture; checks the
