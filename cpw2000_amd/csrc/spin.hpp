@@ -8,6 +8,7 @@
 #pragma once
 
 #include "fft_fast.hpp"
+#include "fft_reg.hpp"
 
 namespace cpwgpu {
 
@@ -194,6 +195,118 @@ __global__ __launch_bounds__(FT) void k_plane_spin(cplx<R> *__restrict__ colbuf,
         const int o = (uy >> 16) * nxp + (uy & 0xffff);
         cb[0][c] = sub[0][o];
         cb[1][c] = sub[1][o];
+    }
+}
+
+// Register variant of k_plane_spin, with the line layout of k_plane_reg: a
+// line of N = G*M points lives in M registers of G adjacent lanes, the
+// cross-lane stages are DPP butterflies.  LDS holds the twiddles twN[N] and the
+// y/x-box sub-planes of both components, sub[2][N][S] (S = nx + 2: box columns,
+// a zero column and a trash column).  The y lines of the two planes are
+// independent tasks; an x-row task takes row y of the up and of the down plane
+// in the same lanes, so after both inverse transforms every lane holds the same
+// M consecutive x points of the two components and applies the 2x2 potential
+// there (vsp = [V1 + V2, V1 - V2, V3, V4], stride vstride), then both forward
+// transforms.  Grid: n3 * npair blocks of T threads, one (z, band pair) each.
+template <typename R, int N, int GX, int GY, int T, int W>
+__global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ colbuf,
+                                                      const int *__restrict__ col_uy,
+                                                      const R *__restrict__ vsp, long vstride,
+                                                      const double2 *__restrict__ twg, FftGeom g,
+                                                      int npair, int S) {
+    static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
+    constexpr int MX = N / GX, MY = N / GY;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<R> *twN = reinterpret_cast<cplx<R> *>(smem_raw);  // [N]
+    cplx<R> *sub = twN + N;                                 // [2][N][S]: up, down
+    const int n3 = g.n[2];
+    const int z = blockIdx.x % n3, pair = blockIdx.x / n3;
+    const int nx = g.nx, kd1 = g.kd[0], ushift = N - nx, xhi = N - kd1;
+    const int tid = threadIdx.x;
+
+    load_twiddles(twN, twg, N);
+    for (int q = tid; q < 2 * N * S; q += T) sub[q] = mk<R>(0, 0);
+    __syncthreads();
+    cplx<R> *cb[2];
+    cb[0] = colbuf + ((long)pair * n3 + z) * g.ncolp;
+    cb[1] = colbuf + ((long)(pair + npair) * n3 + z) * g.ncolp;
+    for (int c = tid; c < g.ncol; c += T) {
+        const int uy = col_uy[c];
+        const int o = (uy >> 16) * S + (uy & 0xffff);
+        sub[o] = cb[0][c];
+        sub[N * S + o] = cb[1][c];
+    }
+    __syncthreads();
+
+    {  // inverse y on the 2 * nx box lines
+        const Lanes<R, GY> L;
+        for (int task = tid / GY; task < 2 * nx; task += T / GY) {
+            const int p = task >= nx ? 1 : 0, u = task - p * nx;
+            cplx<R> *col = sub + p * N * S + u;
+            cplx<R> v[MY];
+#pragma unroll
+            for (int j = 0; j < MY; j++) v[j] = col[(L.g + GY * j) * S];
+            line_dif<+1, GY, MY>(v, L, twN);
+#pragma unroll
+            for (int k = 0; k < MY; k++) col[(k + MY * L.k2) * S] = v[k];
+        }
+    }
+    __syncthreads();
+
+    {  // rows: inverse x of both components, the 2x2 potential, forward x
+        const Lanes<R, GX> L;
+        for (int y = tid / GX; y < N; y += T / GX) {
+            cplx<R> *ru = sub + y * S, *rd = sub + (N + y) * S;
+            cplx<R> u[MX], d[MX];
+#pragma unroll
+            for (int j = 0; j < MX; j++) {
+                const int x = L.g + GX * j;
+                const int o = x <= kd1 ? x : x >= xhi ? x - ushift : nx;
+                u[j] = ru[o];
+                d[j] = rd[o];
+            }
+            line_dif<+1, GX, MX>(u, L, twN);
+            line_dif<+1, GX, MX>(d, L, twN);
+            const R *v0 = vsp + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
+#pragma unroll
+            for (int k = 0; k < MX; k++) {
+                const R vu = v0[k], vd = v0[vstride + k], vx = v0[2 * vstride + k], vy = v0[3 * vstride + k];
+                const cplx<R> a = u[k], b = d[k];
+                u[k] = mk<R>(vu * a.x + vx * b.x + vy * b.y, vu * a.y + vx * b.y - vy * b.x);
+                d[k] = mk<R>(vd * b.x + vx * a.x - vy * a.y, vd * b.y + vx * a.y + vy * a.x);
+            }
+            line_dit<-1, GX, MX>(u, L, twN);
+            line_dit<-1, GX, MX>(d, L, twN);
+#pragma unroll
+            for (int m = 0; m < MX; m++) {
+                const int x = L.g + GX * m;
+                const int o = x <= kd1 ? x : x >= xhi ? x - ushift : nx + 1;
+                ru[o] = u[m];
+                rd[o] = d[m];
+            }
+        }
+    }
+    __syncthreads();
+
+    {  // forward y
+        const Lanes<R, GY> L;
+        for (int task = tid / GY; task < 2 * nx; task += T / GY) {
+            const int p = task >= nx ? 1 : 0, u = task - p * nx;
+            cplx<R> *col = sub + p * N * S + u;
+            cplx<R> v[MY];
+#pragma unroll
+            for (int k = 0; k < MY; k++) v[k] = col[(k + MY * L.k2) * S];
+            line_dit<-1, GY, MY>(v, L, twN);
+#pragma unroll
+            for (int m = 0; m < MY; m++) col[(L.g + GY * m) * S] = v[m];
+        }
+    }
+    __syncthreads();
+    for (int c = tid; c < g.ncol; c += T) {
+        const int uy = col_uy[c];
+        const int o = (uy >> 16) * S + (uy & 0xffff);
+        cb[0][c] = sub[o];
+        cb[1][c] = sub[N * S + o];
     }
 }
 

@@ -54,7 +54,7 @@ EXPORTS = (
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
-TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN = 0, 1, 2, 3, 4
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN = 0, 1, 2, 3, 4, 5
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
@@ -333,9 +333,10 @@ class Plan:
         (projection GEMM, 0 LDS-staged, 1..4 direct-load shapes, 5 LDS-staged
         with half-size wave tiles), TUNE_BACK (back-projection GEMM, 0
         LDS-staged, 1..3 direct-load shapes, 4 LDS-staged with half-size chunks),
-        TUNE_SPLITK (projection GEMM split-K target blocks, -1 default) or
+        TUNE_SPLITK (projection GEMM split-K target blocks, -1 default),
         TUNE_KIN (kinetic term in the back-projection epilogue 0 or in the
-        column gather 1)."""
+        column gather 1) or TUNE_SPIN (plane kernel of apply_spin with nsp = 4:
+        0 LDS two-pass, 1 register lines where built, -1 default)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 
     def get_variant(self, prec, what):

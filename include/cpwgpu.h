@@ -163,12 +163,17 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *   CPWGPU_TUNE_KIN     where ekpg*psi is added when projectors are set: 0 = in
  *                       the back-projection epilogue, 1 = in the gather of the
  *                       forward column kernel (moves work between the two
- *                       streams); -1 = default (0) */
+ *                       streams); -1 = default (0)
+ *   CPWGPU_TUNE_SPIN    plane kernel of cpwgpu_plan_apply_spin with nsp = 4: 0 =
+ *                       LDS two-pass lines, 1 = register-resident lines (built
+ *                       when 8 divides the mesh length); -1 = default (1
+ *                       where built, else 0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2
 #define CPWGPU_TUNE_BACK 3
 #define CPWGPU_TUNE_KIN 4
+#define CPWGPU_TUNE_SPIN 5
 int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
 /* Current choice (the value used by the next apply, defaults resolved). */
 int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);

@@ -4,7 +4,8 @@ to the scalar oracle, a dense G-space spinor H and the Pauli form).
 
 FP64: relative 2-norm error <= 1e-12; FP32-mixed: <= 2e-6.  nsp = 1, 2, 4
 on the 2-atom cell (24^3) through the device API, nsp = 4 through the
-host drop-in and on the 64-atom cell (72^3)."""
+host drop-in and on the 64-atom cell (72^3); nsp = 4 with both coupled plane
+kernels (LDS two-pass and register lines) in both precisions."""
 
 import numpy as np
 import pytest
@@ -43,6 +44,17 @@ def _plan(hk, d, s, nsp):
     return plan
 
 
+def _spin_kernels(hk, plan, nsp):
+    """Coupled plane kernels to run: both for nsp = 4 (the register one is
+    built on these meshes), the default otherwise."""
+    if nsp != 4:
+        return [-1]
+    for prec in (hk.PREC_FP64, hk.PREC_FP32):
+        plan.set_variant(prec, hk.TUNE_SPIN, 1)
+        assert plan.get_variant(prec, hk.TUNE_SPIN) == 1
+    return [0, 1]
+
+
 @pytest.mark.parametrize("nsp", [1, 2, 4])
 def test_apply_spin_matches_oracle(torch_dev, hk, nsp):
     torch = torch_dev
@@ -52,11 +64,13 @@ def test_apply_spin_matches_oracle(torch_dev, hk, nsp):
     ref = spin.hk_psi_spin(d, s["psi_sp"], s["vscr_sp"], nsp, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
     plan = _plan(hk, d, s, nsp)
     pt = torch.from_numpy(np.ascontiguousarray(s["psi_sp"][: 2 * mtxd].T)).to("cuda")
-    for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
-        ht = torch.zeros_like(pt)
-        plan.apply_spin(pt, ht, prec)
-        torch.cuda.synchronize()
-        assert _rel(ht.cpu().numpy().T, ref) < tol, (nsp, prec)
+    for kern in _spin_kernels(hk, plan, nsp):
+        for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
+            plan.set_variant(prec, hk.TUNE_SPIN, kern)
+            ht = torch.zeros_like(pt)
+            plan.apply_spin(pt, ht, prec)
+            torch.cuda.synchronize()
+            assert _rel(ht.cpu().numpy().T, ref) < tol, (nsp, kern, prec)
     # without projectors: local + kinetic only
     plan.set_projectors_spin(s["anlsp"], s["xnlkbsp"], 0)
     ref0 = spin.hk_psi_spin(d, s["psi_sp"], s["vscr_sp"], nsp)
@@ -97,11 +111,13 @@ def test_noncollinear_si64(torch_dev, hk):
     ref = spin.hk_psi_spin(d, s["psi_sp"], s["vscr_sp"], 4, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
     plan = _plan(hk, d, s, 4)
     pt = torch.from_numpy(np.ascontiguousarray(s["psi_sp"][: 2 * mtxd].T)).to("cuda")
-    for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
-        ht = torch.zeros_like(pt)
-        plan.apply_spin(pt, ht, prec)
-        torch.cuda.synchronize()
-        assert _rel(ht.cpu().numpy().T, ref) < tol, prec
+    for kern in _spin_kernels(hk, plan, 4):
+        for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
+            plan.set_variant(prec, hk.TUNE_SPIN, kern)
+            ht = torch.zeros_like(pt)
+            plan.apply_spin(pt, ht, prec)
+            torch.cuda.synchronize()
+            assert _rel(ht.cpu().numpy().T, ref) < tol, (kern, prec)
     plan.close()
 
 

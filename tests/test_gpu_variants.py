@@ -124,11 +124,18 @@ def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (7, 0)):
+                        (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (hk.TUNE_SPIN, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
         plan.set_variant(5, hk.TUNE_PLANE, 0)
+    # the register two-component kernel needs a specialised mesh (12^3 is not)
+    dg = synth.make_config(0, nbands=2, kmax_override=[5, 5, 5])
+    pg = hk.Plan.from_inputs(dg)
+    with pytest.raises(hk.CpwGpuError):
+        pg.set_variant(hk.PREC_FP32, hk.TUNE_SPIN, 1)
+    assert pg.get_variant(hk.PREC_FP32, hk.TUNE_SPIN) == 0
+    pg.close()
     plan.close()
 
 
