@@ -1837,17 +1837,22 @@ void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int split
     std::fclose(f);
 }
 
-// median-free timing: the best of `reps` applies after one warm-up
+// median-free timing: the best of `reps` applies (two-component ones with
+// spin) after one warm-up
 int time_apply(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-               int prec, hipStream_t s, int reps, float *best) {
+               int prec, hipStream_t s, int reps, float *best, bool spin = false) {
+    auto apply = [&]() {
+        return spin ? cpwgpu_plan_apply_spin(p, neig, psi, ldp, hpsi, ldh, prec, s)
+                    : cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+    };
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    int st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+    int st = apply();
     *best = 3.4e38f;
     for (int r = 0; r < reps && !st; r++) {
         HIPCHK(hipEventRecord(a, s));
-        st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
+        st = apply();
         HIPCHK(hipEventRecord(b, s));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0.f;
@@ -1929,6 +1934,52 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
     tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv);
     return CPWGPU_OK;
+}
+
+// Tuner of the two-component path: times cpwgpu_plan_apply_spin for the coupled
+// plane kernels (nsp = 4), then the projection kernel x split-K target, then
+// the back-projection kernel, on the spinor GEMM shapes (2 mtxd rows, nanlsp
+// projectors).  The GEMM choices are the plan's (shared with the scalar path).
+int tune_spin(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh, int prec,
+              hipStream_t s) {
+    const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
+    int st;
+    float best, t;
+    if (p->nsp == 4) {
+        const bool built = prec == CPWGPU_PREC_FP32 ? spin_reg_built<float>(p) : spin_reg_built<double>(p);
+        int best_sv = 0;
+        best = 3.4e38f;
+        for (int sv = 0; sv <= (built ? 1 : 0); sv++) {
+            p->spin_var[ix] = sv;
+            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t, true))) return st;
+            if (t < best) { best = t; best_sv = sv; }
+        }
+        p->spin_var[ix] = best_sv;
+    }
+    if (p->snanl > 0) {
+        const int cand[4] = {128, 256, 512, 1024};
+        int best_jv = 0, best_sk = -1;
+        best = 3.4e38f;
+        for (int jv = 0; jv < NPROJ_VAR; jv++)
+            for (int c : cand) {
+                p->proj_var[ix] = jv;
+                p->splitk[ix] = c;
+                if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t, true))) return st;
+                if (t < best) { best = t; best_sk = c; best_jv = jv; }
+            }
+        p->proj_var[ix] = best_jv;
+        p->splitk[ix] = best_sk;
+        int best_bv = 0;
+        best = 3.4e38f;
+        for (int bv = 0; bv < NBACK_VAR; bv++) {
+            p->back_var[ix] = bv;
+            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t, true))) return st;
+            if (t < best) { best = t; best_bv = bv; }
+        }
+        p->back_var[ix] = best_bv;
+    }
+    // leave hpsi holding H psi of the chosen configuration
+    return cpwgpu_plan_apply_spin(p, neig, psi, ldp, hpsi, ldh, prec, s);
 }
 
 }  // namespace
@@ -2301,6 +2352,14 @@ int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long l
     if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
     if (neig <= 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
     return tune_plan(plan, neig, psi_dev, ldp, hpsi_dev, ldh, prec, (hipStream_t)stream);
+}
+
+int cpwgpu_plan_tune_spin(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                          long ldh, int prec, void *stream) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    if (neig <= 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
+    return tune_spin(plan, neig, psi_dev, ldp, hpsi_dev, ldh, prec, (hipStream_t)stream);
 }
 
 int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out) {

@@ -47,7 +47,7 @@ EXPORTS = (
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
     "cpwgpu_profile", "cpwgpu_profile_read", "cpwgpu_rayleigh_ritz", "cpwgpu_plan_forces",
     "cpwgpu_graph_create", "cpwgpu_graph_launch", "cpwgpu_graph_destroy",
-    "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune",
+    "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune", "cpwgpu_plan_tune_spin",
     "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
     "cpwgpu_hk_psi_spin_c16", "cpwgpu_plan_set_potential_spin", "cpwgpu_plan_set_projectors_spin",
     "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8",
@@ -415,6 +415,22 @@ class Plan:
                                             _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
                                             int(prec), C.c_void_p(stream)))
         return hpsi_t
+
+    def tune_spin(self, psi_t, hpsi_t, prec=PREC_FP32, stream=None):
+        """Autotune the two-component path on these device arrays (as
+        apply_spin; hpsi_t receives H psi); returns the choice {"spin",
+        "proj", "back", "splitk"}."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        _check(lib().cpwgpu_plan_tune_spin(self._h, int(psi_t.shape[0]), C.c_void_p(psi_t.data_ptr()),
+                                           _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t), int(prec),
+                                           C.c_void_p(stream)))
+        return {"spin": self.get_variant(prec, TUNE_SPIN),
+                "proj": self.get_variant(prec, TUNE_PROJ),
+                "back": self.get_variant(prec, TUNE_BACK),
+                "splitk": self.get_variant(prec, TUNE_SPLITK)}
 
     def density(self, psi_t, occ=None, stream=None):
         """rho(r) = sum_n occ_n |psi_n(r)|^2 on the mesh (cpwgpu_plan_density):

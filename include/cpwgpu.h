@@ -186,6 +186,14 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);
  * tuning; the key is (precision, n1, n2, n3, nx, ncol, nanl, neig). */
 int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
                      long ldh, int prec, void *stream);
+/* Autotuner of the two-component path (cpwgpu_plan_apply_spin arguments; the
+ * potential and projectors of cpwgpu_plan_set_*_spin must be set): times the
+ * coupled plane kernels (nsp = 4), the projection kernel with a few split-K
+ * targets and the back-projection kernel on the spinor GEMM shapes, and keeps
+ * the fastest in the plan (the GEMM choices are shared with the scalar path).
+ * hpsi_dev is overwritten with H psi.  No cache file. */
+int cpwgpu_plan_tune_spin(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
+                          long ldh, int prec, void *stream);
 
 /* ---- several k-points in one graph launch -------------------------------------- */
 

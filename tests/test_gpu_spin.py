@@ -81,6 +81,35 @@ def test_apply_spin_matches_oracle(torch_dev, hk, nsp):
     plan.close()
 
 
+def test_tune_spin_keeps_results(torch_dev, hk):
+    """The two-component tuner picks valid kernels and leaves H psi of the
+    chosen configuration in hpsi; later applies agree with the oracle."""
+    torch = torch_dev
+    d = synth.make_config(0, nbands=2)
+    s = synth.spin_inputs(d, 6)
+    mtxd = d["mtxd"]
+    ref = spin.hk_psi_spin(d, s["psi_sp"], s["vscr_sp"], 4, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
+    plan = _plan(hk, d, s, 4)
+    pt = torch.from_numpy(np.ascontiguousarray(s["psi_sp"][: 2 * mtxd].T)).to("cuda")
+    for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
+        ht = torch.zeros_like(pt)
+        choice = plan.tune_spin(pt, ht, prec)
+        torch.cuda.synchronize()
+        assert choice["spin"] in (0, 1) and 0 <= choice["proj"] <= 5 and 0 <= choice["back"] <= 4
+        assert choice["splitk"] in (128, 256, 512, 1024)
+        assert _rel(ht.cpu().numpy().T, ref) < tol, (prec, choice)
+        h2 = torch.zeros_like(pt)
+        plan.apply_spin(pt, h2, prec)
+        torch.cuda.synchronize()
+        assert _rel(h2.cpu().numpy().T, ref) < tol, (prec, choice)
+    # the potential must be set first
+    plain = hk.Plan.from_inputs(d)
+    with pytest.raises(hk.CpwGpuError):
+        plain.tune_spin(pt, torch.zeros_like(pt), hk.PREC_FP32)
+    plain.close()
+    plan.close()
+
+
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32mixed"])
 def test_hk_psi_spin_c16_drop_in(hk, prec):
     """The reference's argument list, host arrays, mxddim > mtxd rows."""
