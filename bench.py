@@ -345,6 +345,27 @@ def bench_spin_density(torch, hk, synth, np, d, steps):
     return out
 
 
+def bench_forces(torch, hk, synth, np, steps):
+    """Atomic forces at fixed psi (adjoint of E_loc + E_NL, cpwgpu_plan_forces)
+    on the two-species 64-atom Si/Ge cell with 128 occupied bands: per-call
+    time including the band density, its transform to the potential's q list,
+    the structure-factor sums and the weighted projection GEMMs; host results."""
+    d = synth.make_config(4)
+    plan = hk.Plan.from_inputs(d)
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda")
+    args = (d["rat"], d["avec"], d["proj_atom"], d["floc_q"], d["floc_f"], d["species"])
+    for _ in range(2):
+        f, _ = plan.forces(psi, *args)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f, enl = plan.forces(psi, *args)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    plan.close()
+    return {"workload": "Si/Ge 64 atoms L=21.0 bohr, Ecut 15 Ha, 128 bands", "atoms": int(d["rat"].shape[0]),
+            "bands": int(psi.shape[0]), "ms_per_call": round(ms, 3),
+            "max_force_ha_bohr": round(float(np.abs(f).max()), 6)}
+
+
 def bench_cfg2(torch, hk, synth, np, steps):
     """216-atom Si (108^3 mesh, 864 projectors, 512 bands): throughput of both
     paths and the FP32-mixed error against FP64 (H psi norm and Ritz values of
@@ -632,6 +653,8 @@ def main():
         out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))
     log("bench: two-component bands and the density")
     out["spinor_density"] = bench_spin_density(torch, hk, synth, np, d, max(5, args.steps // 3))
+    log("bench: atomic forces (Si/Ge 64 atoms, 128 bands)")
+    out["forces"] = bench_forces(torch, hk, synth, np, 5)
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
