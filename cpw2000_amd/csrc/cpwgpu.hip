@@ -1078,6 +1078,34 @@ struct Subspace {
     DevBuf s, m, w, v, part, pos, rot, info, eig, res, tmp;
 } g_ss;
 
+// The subspace products use the direct-load FP64 GEMM shapes that measured
+// fastest on the 64-atom cell (projection shape 2 at 0.72 and back-projection
+// shape 2 at 0.67 of the FP64 matrix peak, against 0.35 / 0.49 for the
+// LDS-staged kernels).
+constexpr int SUB_PROJ_V = 2, SUB_BACK_V = 2;
+
+// split-K partials of a^H b (na x nbc over mtxd rows) into part; returns the
+// number of slices in *nsl
+int gram_partials(const double2 *a, long lda, int na, const double2 *b, long ldb, int nbc, int mtxd,
+                  DevBuf &part, int *nsl_out, hipStream_t st, int lower = 0) {
+    constexpr int BM = direct_bm<double, SUB_PROJ_V>(), BN = direct_bn<double, SUB_PROJ_V>();
+    constexpr int BK = direct_bk<double, SUB_PROJ_V>();
+    const int tm = (na + BM - 1) / BM, tn = (nbc + BN - 1) / BN;
+    const int ksteps = (mtxd + BK - 1) / BK;
+    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
+    // slices are whole K chunks of the kernel
+    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+    nsl = (mtxd + kslice - 1) / kslice;
+    int rc;
+    if ((rc = part.reserve(16 * (size_t)nsl * na * nbc))) return rc;
+    hipLaunchKernelGGL((k_proj_direct<double, SUB_PROJ_V>), dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
+                       (const cplx<double> *)a, lda, b, ldb, (cplx<double> *)part.p, mtxd, na, nbc, kslice,
+                       lower);
+    HIPCHK(hipGetLastError());
+    *nsl_out = nsl;
+    return CPWGPU_OK;
+}
+
 int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long ldh, int rotate,
                   double *eig_h, double *res_h, hipStream_t st) {
     int rc;
@@ -1091,21 +1119,13 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
         (rc = g_ss.res.reserve(8 * (size_t)nb)))
         return rc;
     // Gram matrices on the matrix cores: split-K over the plane waves
-    constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
-    const int tm = (nb + BM - 1) / BM, tn = (nb + BN - 1) / BN;
-    const int ksteps = (mtxd + BK - 1) / BK;
-    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
-    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
-    nsl = (mtxd + kslice - 1) / kslice;
-    if ((rc = g_ss.part.reserve(16 * (size_t)nsl * nb * nb))) return rc;
     double2 *S = (double2 *)g_ss.s.p, *M = (double2 *)g_ss.m.p, *W = (double2 *)g_ss.w.p,
             *V = (double2 *)g_ss.v.p;
     for (int which = 0; which < 2; which++) {
         const double2 *y = which == 0 ? psi : hpsi;
         const long ldy = which == 0 ? ldp : ldh;
-        hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
-                           (const cplx<double> *)psi, ldp, y, ldy, (cplx<double> *)g_ss.part.p, mtxd,
-                           nb, nb, kslice);
+        int nsl;
+        if ((rc = gram_partials(psi, ldp, nb, y, ldy, nb, mtxd, g_ss.part, &nsl, st))) return rc;
         launch_proj_reduce<double>((const cplx<double> *)g_ss.part.p, nullptr,
                                    (cplx<double> *)(which == 0 ? S : M), nb, nb, nsl, st);
     }
@@ -1130,9 +1150,11 @@ int rayleigh_ritz(int mtxd, int nb, double2 *psi, long ldp, double2 *hpsi, long 
         for (int which = 0; which < 2; which++) {
             double2 *x = which == 0 ? psi : hpsi;
             const long ldx = which == 0 ? ldp : ldh;
-            hipLaunchKernelGGL(k_back<double>, dim3((mtxd + BP_BI - 1) / BP_BI, (nb + BP_BN - 1) / BP_BN),
+            constexpr int BI = back_bi<double, SUB_BACK_V>(), BN = back_bn<double, SUB_BACK_V>();
+            hipLaunchKernelGGL((k_back_direct<double, SUB_BACK_V>), dim3((mtxd + BI - 1) / BI, (nb + BN - 1) / BN),
                                dim3(GEMM_THREADS), 0, st, (const cplx<double> *)x, ldx,
-                               (const cplx<double> *)V, T, (long)mtxd, mtxd, nb, nb, 0);
+                               (const cplx<double> *)V, T, (long)mtxd, mtxd, nb, nb, 0,
+                               (const double2 *)nullptr, 0L, (const double *)nullptr);
             hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
                                (const double2 *)T, (long)mtxd, x, ldx, mtxd, nb);
         }
@@ -1200,17 +1222,8 @@ constexpr unsigned long long DT_SEED = 0x243F6A8885A308D3ull;
 // lower triangle is computed, the strict upper one is set to zero.
 int gram(const double2 *a, long lda, int na, const double2 *b, long ldb, int nbc, int mtxd,
          const double *scale, double2 *out, hipStream_t st, int lower = 0) {
-    constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
-    const int tm = (na + BM - 1) / BM, tn = (nbc + BN - 1) / BN;
-    const int ksteps = (mtxd + BK - 1) / BK;
-    int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
-    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
-    nsl = (mtxd + kslice - 1) / kslice;
-    int rc;
-    if ((rc = g_ss.part.reserve(16 * (size_t)nsl * na * nbc))) return rc;
-    hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
-                       (const cplx<double> *)a, lda, b, ldb, (cplx<double> *)g_ss.part.p, mtxd, na,
-                       nbc, kslice, nullptr, lower);
+    int nsl, rc;
+    if ((rc = gram_partials(a, lda, na, b, ldb, nbc, mtxd, g_ss.part, &nsl, st, lower))) return rc;
     launch_proj_reduce<double>((const cplx<double> *)g_ss.part.p, scale, (cplx<double> *)out, na,
                                nbc, nsl, st, lower);
     HIPCHK(hipGetLastError());
@@ -1220,9 +1233,10 @@ int gram(const double2 *a, long lda, int na, const double2 *b, long ldb, int nbc
 // y(:, 0:nc) = beta*y + x(:, 0:nk) c; c column-major (nk, nc) with ld nk; beta 0 or 1
 int rotate(const double2 *x, long ldx, int nk, const double2 *c, int nc, double2 *y, long ldy,
            int mtxd, int beta, hipStream_t st) {
-    hipLaunchKernelGGL(k_back<double>, dim3((mtxd + BP_BI - 1) / BP_BI, (nc + BP_BN - 1) / BP_BN),
-                       dim3(GEMM_THREADS), 0, st, (const cplx<double> *)x, ldx,
-                       (const cplx<double> *)c, y, ldy, mtxd, nk, nc, beta);
+    constexpr int BI = back_bi<double, SUB_BACK_V>(), BN = back_bn<double, SUB_BACK_V>();
+    hipLaunchKernelGGL((k_back_direct<double, SUB_BACK_V>), dim3((mtxd + BI - 1) / BI, (nc + BN - 1) / BN),
+                       dim3(GEMM_THREADS), 0, st, (const cplx<double> *)x, ldx, (const cplx<double> *)c, y,
+                       ldy, mtxd, nk, nc, beta, (const double2 *)nullptr, 0L, (const double *)nullptr);
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
 }

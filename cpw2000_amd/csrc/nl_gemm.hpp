@@ -279,12 +279,15 @@ __device__ __forceinline__ void load_psi_chunk(double2 (&dst)[KPL], const double
 template <typename R, int V>
 __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_direct(
     const cplx<R> *__restrict__ anl, long lda, const double2 *__restrict__ psi, long ldp,
-    cplx<R> *__restrict__ part, int mtxd, int nanl, int neig, int kslice) {
+    cplx<R> *__restrict__ part, int mtxd, int nanl, int neig, int kslice, int lower = 0) {
     typedef typename MF<R>::v4 v4;
     constexpr int WM = DirectCfg<R, V>::WM, WN = DirectCfg<R, V>::WN;
     constexpr int KPL = DirectCfg<R, V>::KPL, KC = 4 * KPL;
     constexpr int BM = direct_bm<R, V>(), BN = direct_bn<R, V>();
     const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    // lower != 0: only the lower triangle (a >= n) of a Hermitian Gram matrix;
+    // tiles wholly above the diagonal are skipped (the reduction zeroes them)
+    if (lower && a0 + BM <= n0) return;
     const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
