@@ -224,7 +224,7 @@ def bench_kpoints(torch, hk, synth, np, steps):
         c = plans[0].tune(psis[0], outs[0], pid)
         for p in plans[1:]:
             for what, key in ((hk.TUNE_PLANE, "plane"), (hk.TUNE_PROJ, "proj"),
-                              (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk")):
+                              (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk"), (hk.TUNE_KIN, "kin")):
                 p.set_variant(pid, what, c[key])
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -516,6 +516,9 @@ def main():
     ap.add_argument("--variant-sweep", action="store_true",
                     help="only time every projection / back-projection kernel alone (one "
                          "stream), then the calibration copy; prints their JSON and exits")
+    ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints"),
+                    help="run only this section (after tuning the 64-atom plan) and print its JSON; "
+                         "for profiling one path at a time")
     ap.add_argument("--issue-rates", action="store_true",
                     help="only measure the issue cost of the FFT kernels' VALU instruction kinds "
                          "(cycles per wave instruction per SIMD) and the shader clock; prints JSON")
@@ -555,9 +558,22 @@ def main():
 
     # autotuner: plane-stage kernel and split-K target per precision, timed on
     # this workload (part of the warm-up; every later apply uses the choice)
+    if args.only in ("spinor", "forces", "kpoints"):
+        # these sections build and tune their own plans
+        steps = max(5, args.steps // 3)
+        sec = {"spinor": lambda: bench_spin_density(torch, hk, synth, np, d, steps),
+               "forces": lambda: bench_forces(torch, hk, synth, np, 5),
+               "kpoints": lambda: bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))}[args.only]
+        print(json.dumps({args.only: sec()}), flush=True)
+        plan.close()
+        return
     tuned = {}
     for prec, pid, h in (("fp64", hk.PREC_FP64, h64), ("fp32", hk.PREC_FP32, h32)):
         tuned[prec] = plan.tune(psi, h, pid)
+    if args.only == "ditsp":
+        print(json.dumps({"ditsp": bench_ditsp(torch, hk, plan, d, nb, np), "tuned": tuned}), flush=True)
+        plan.close()
+        return
     log(f"bench: tuned kernels {tuned}")
 
     # accuracy of the FP32-mixed path against FP64 (same inputs, same run)
