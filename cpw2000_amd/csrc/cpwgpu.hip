@@ -1084,10 +1084,10 @@ struct Subspace {
 // LDS-staged kernels).
 constexpr int SUB_PROJ_V = 2, SUB_BACK_V = 2;
 
-// split-K partials of a^H b (na x nbc over mtxd rows) into part; returns the
-// number of slices in *nsl
+// split-K partials of a^H W b (na x nbc over mtxd rows, W = diag(wrow) or
+// the identity) into part; returns the number of slices in *nsl
 int gram_partials(const double2 *a, long lda, int na, const double2 *b, long ldb, int nbc, int mtxd,
-                  DevBuf &part, int *nsl_out, hipStream_t st, int lower = 0) {
+                  DevBuf &part, int *nsl_out, hipStream_t st, int lower = 0, const double *wrow = nullptr) {
     constexpr int BM = direct_bm<double, SUB_PROJ_V>(), BN = direct_bn<double, SUB_PROJ_V>();
     constexpr int BK = direct_bk<double, SUB_PROJ_V>();
     const int tm = (na + BM - 1) / BM, tn = (nbc + BN - 1) / BN;
@@ -1100,7 +1100,7 @@ int gram_partials(const double2 *a, long lda, int na, const double2 *b, long ldb
     if ((rc = part.reserve(16 * (size_t)nsl * na * nbc))) return rc;
     hipLaunchKernelGGL((k_proj_direct<double, SUB_PROJ_V>), dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, st,
                        (const cplx<double> *)a, lda, b, ldb, (cplx<double> *)part.p, mtxd, na, nbc, kslice,
-                       lower);
+                       wrow, lower);
     HIPCHK(hipGetLastError());
     *nsl_out = nsl;
     return CPWGPU_OK;
@@ -1746,18 +1746,13 @@ int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const 
             occ = (const double *)g_fw.occ.p;
         }
         HIPCHK(hipStreamSynchronize(g_stream));
-        constexpr int BM = proj_bm<double>(), BN = proj_bn<double>(), BK = GemmCfg<double>::BK;
-        const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
-        const int ksteps = (g.mtxd + BK - 1) / BK;
-        int nsl = std::max(1, std::min(ksteps, (256 + tm * tn - 1) / (tm * tn)));
-        const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
-        nsl = (g.mtxd + kslice - 1) / kslice;
-        if ((rc = p->part.reserve(16 * (size_t)nsl * mat))) return rc;
         for (int j = 0; j < 4; j++) {
+            // <beta|psi>, then <beta|g_j psi> for the three Cartesian components
             const double *w = j == 0 ? nullptr : (const double *)p->gw.p + (size_t)(j - 1) * g.mtxd;
-            hipLaunchKernelGGL(k_proj<double>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s,
-                               (const cplx<double> *)p->anl64.p, p->lda, psi, ldp,
-                               (cplx<double> *)p->part.p, g.mtxd, nanl, neig, kslice, w);
+            int nsl;
+            if ((rc = gram_partials((const double2 *)p->anl64.p, p->lda, nanl, psi, ldp, neig, g.mtxd, p->part,
+                                    &nsl, s, 0, w)))
+                return rc;
             launch_proj_reduce<double>((const cplx<double> *)p->part.p, nullptr,
                                        (cplx<double> *)g_fw.p.p + (size_t)j * mat, nanl, neig, nsl, s);
         }

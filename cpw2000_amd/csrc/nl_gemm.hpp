@@ -279,7 +279,8 @@ __device__ __forceinline__ void load_psi_chunk(double2 (&dst)[KPL], const double
 template <typename R, int V>
 __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_direct(
     const cplx<R> *__restrict__ anl, long lda, const double2 *__restrict__ psi, long ldp,
-    cplx<R> *__restrict__ part, int mtxd, int nanl, int neig, int kslice, int lower = 0) {
+    cplx<R> *__restrict__ part, int mtxd, int nanl, int neig, int kslice,
+    const double *__restrict__ wrow = nullptr, int lower = 0) {
     typedef typename MF<R>::v4 v4;
     constexpr int WM = DirectCfg<R, V>::WM, WN = DirectCfg<R, V>::WN;
     constexpr int KPL = DirectCfg<R, V>::KPL, KC = 4 * KPL;
@@ -323,6 +324,15 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
         const int i0 = k0 + lg * KPL;
 #pragma unroll
         for (int t = 0; t < WM; t++) load_row_chunk<R, KPL>(ra[t], arow[t], i0, ke, mtxd, aok[t]);
+        if (wrow) {
+            // per-plane-wave weight w_i on the projector operand (zeros stay zero)
+#pragma unroll
+            for (int u = 0; u < KPL; u++) {
+                const R w = (R)wrow[min(i0 + u, mtxd - 1)];
+#pragma unroll
+                for (int t = 0; t < WM; t++) ra[t][u] = scale(ra[t][u], w);
+            }
+        }
 #pragma unroll
         for (int t = 0; t < WN; t++) load_psi_chunk<KPL>(rb[t], bcol[t], i0, ke, mtxd, bok[t]);
     };
