@@ -1618,10 +1618,39 @@ struct ForceWs {
     DevBuf t2, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p, rhor;
 } g_fw;
 
+// register shape of the density pass: x lanes, y lanes, threads, waves per SIMD
+constexpr int DEN_GX = 8, DEN_GY = 8, DEN_T = 256, DEN_W = 3;
+// built when the line split divides N and a lane's row accumulators (rows per
+// x task x N / GX doubles) stay within 27: spill-free at 3 waves per SIMD for
+// 24, 48, 64 and 72 (80, 96 and 128 spill; they keep k_density_fast)
+template <int N> constexpr bool den_reg_ok() {
+    return reg_ok<double, N, DEN_GX, DEN_GY>() &&
+           ((N + DEN_T / DEN_GX - 1) / (DEN_T / DEN_GX)) * (N / DEN_GX) <= 27;
+}
+
 template <int N>
 int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *occ, int nb, int bpg,
                    int ngrp, int Q1, int Q2, double2 *t2, hipStream_t s, double *rhor = nullptr) {
     const FftGeom &g = p->g;
+    if constexpr (den_reg_ok<N>()) {
+        if (rhor) {
+            // the density planes alone: register lines, accumulation in registers
+            const int S = reg_stride(g.nx);
+            const size_t dyn = 16 * ((size_t)N + (size_t)N * S);
+            if (dyn > 160 * 1024) return fail(CPWGPU_ERR_SIZE, "density plane %d does not fit in LDS", N);
+            static bool attr_r = false;
+            if (!attr_r) {
+                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DEN_T, DEN_W>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                attr_r = true;
+            }
+            hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DEN_T, DEN_W>), dim3(g.n[2] * ngrp), dim3(DEN_T),
+                               dyn, s, colbuf, (const int *)p->col_uy.p, occ, (const double2 *)p->tw[0].p, g, nb,
+                               bpg, S, rhor);
+            HIPCHK(hipGetLastError());
+            return CPWGPU_OK;
+        }
+    }
     constexpr int RB = plane_rows_fast<N>();
     const size_t dyn = 16 * ((size_t)N * (g.nx | 1) + (size_t)RB * row_stride<N>()) + 8 * (size_t)N * N;
     const size_t stat = 16 * 2 * (size_t)N;

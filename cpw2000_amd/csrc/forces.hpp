@@ -20,6 +20,7 @@
 #pragma once
 
 #include "fft_fast.hpp"
+#include "fft_reg.hpp"
 
 namespace cpwgpu {
 
@@ -133,6 +134,93 @@ __global__ __launch_bounds__(FT) void k_density_fast(const cplx<double> *__restr
             si += a.y * w2.x - a.x * w2.y;
         }
         out[idx] = make_double2(sr, si);
+    }
+}
+
+// Register-line variant of the density pass of k_density_fast (the line
+// layout of k_plane_reg): per (z, band group) each band's y/x-box sub-plane
+// (LDS, S = nx + 2 columns with a zero column) goes through the inverse y and
+// x transforms in registers.  An x-row task keeps the same rows for every band
+// of the group, so occ |psi(x, y)|^2 accumulates in the registers of the lane
+// that holds (x, y) (x = MX k2 + k after the decimation-in-frequency
+// transform), and each block writes its rho(x, y) plane once:
+// rhor[grp][z][y][x], summed over groups in a fixed order afterwards.
+// Grid: n3 * ceil(nb / bpg) blocks of T threads.
+template <int N, int GX, int GY, int T, int W>
+__global__ __launch_bounds__(T, W) void k_density_reg(const cplx<double> *__restrict__ colbuf,
+                                                   const int *__restrict__ col_uy,
+                                                   const double *__restrict__ occ,
+                                                   const double2 *__restrict__ twg, FftGeom g, int nb,
+                                                   int bpg, int S, double *__restrict__ rhor) {
+    typedef double R;
+    static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
+    constexpr int MX = N / GX, MY = N / GY;
+    constexpr int TX = T / GX;                 // x-row tasks per round
+    constexpr int RPT = (N + TX - 1) / TX;     // rows per x task
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<R> *twN = reinterpret_cast<cplx<R> *>(smem_raw);  // [N]
+    cplx<R> *sub = twN + N;                                 // [N][S]
+    const int n3 = g.n[2];
+    const int z = blockIdx.x % n3, grp = blockIdx.x / n3;
+    const int b0 = grp * bpg, b1 = min(nb, b0 + bpg);
+    const int nx = g.nx, kd1 = g.kd[0], ushift = N - nx, xhi = N - kd1;
+    const int tid = threadIdx.x;
+    load_twiddles(twN, twg, N);
+    const Lanes<R, GX> LX;
+    const Lanes<R, GY> LY;
+    R racc[RPT][MX];
+#pragma unroll
+    for (int r = 0; r < RPT; r++)
+#pragma unroll
+        for (int k = 0; k < MX; k++) racc[r][k] = 0.0;
+    for (int b = b0; b < b1; b++) {
+        for (int q = tid; q < N * S; q += T) sub[q] = mk<R>(0, 0);
+        __syncthreads();
+        const cplx<R> *cb = colbuf + ((long)b * n3 + z) * g.ncolp;
+        for (int c = tid; c < g.ncol; c += T) {
+            const int uy = col_uy[c];
+            sub[(uy >> 16) * S + (uy & 0xffff)] = cb[c];
+        }
+        __syncthreads();
+        // inverse y on the nx box lines
+        for (int u = tid / GY; u < nx; u += T / GY) {
+            cplx<R> *col = sub + u;
+            cplx<R> v[MY];
+#pragma unroll
+            for (int j = 0; j < MY; j++) v[j] = col[(LY.g + GY * j) * S];
+            line_dif<+1, GY, MY>(v, LY, twN);
+#pragma unroll
+            for (int k = 0; k < MY; k++) col[(k + MY * LY.k2) * S] = v[k];
+        }
+        __syncthreads();
+        // inverse x per row, occ |psi|^2 into the row's registers
+        const R w = occ ? occ[b] : 1.0;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int y = tid / GX + r * TX;
+            if (y < N) {
+                const cplx<R> *row = sub + y * S;
+                cplx<R> v[MX];
+#pragma unroll
+                for (int j = 0; j < MX; j++) {
+                    const int x = LX.g + GX * j;
+                    v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
+                }
+                line_dif<+1, GX, MX>(v, LX, twN);
+#pragma unroll
+                for (int k = 0; k < MX; k++) racc[r][k] += w * (v[k].x * v[k].x + v[k].y * v[k].y);
+            }
+        }
+        __syncthreads();  // the rows are read before the next band clears sub
+    }
+    double *o = rhor + ((long)grp * n3 + z) * (N * N);
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int y = tid / GX + r * TX;
+        if (y < N) {
+#pragma unroll
+            for (int k = 0; k < MX; k++) o[y * N + MX * LX.k2 + k] = racc[r][k];
+        }
     }
 }
 
