@@ -1615,7 +1615,7 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
 
 // ---- atomic forces ------------------------------------------------------------
 struct ForceWs {
-    DevBuf t2, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p, rhor;
+    DevBuf t2, t2s, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p, rhor;
 } g_fw;
 
 // register shape of the density pass: x lanes, y lanes, threads, waves per SIMD
@@ -1746,9 +1746,14 @@ int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const 
         CPW_FAST_SIZES(CPW_DEN)
 #undef CPW_DEN
         if (rc) return rc;
-        hipLaunchKernelGGL(k_rho_q, dim3((nq + 255) / 256), dim3(256), 0, s, (const double2 *)g_fw.t2.p,
-                           (const int *)g_fw.qv.p, nq, (const double2 *)p->tw[2].p, g.n[2], ngrp, Q1, Q2,
-                           p->inv_n, (double2 *)g_fw.rho.p);
+        // the band groups first (fixed order, all of the GPU), then per q the z sum
+        const long nplane = (long)g.n[2] * (2 * Q2 + 1) * (2 * Q1 + 1);
+        if ((rc = g_fw.t2s.reserve(16 * (size_t)nplane))) return rc;
+        hipLaunchKernelGGL(k_t2_sum, dim3((unsigned)((nplane + 255) / 256)), dim3(256), 0, s,
+                           (const double2 *)g_fw.t2.p, nplane, ngrp, (double2 *)g_fw.t2s.p);
+        hipLaunchKernelGGL(k_rho_q, dim3((nq + 255) / 256), dim3(256), 0, s, (const double2 *)g_fw.t2s.p,
+                           (const int *)g_fw.qv.p, nq, (const double2 *)p->tw[2].p, g.n[2], Q1, Q2, p->inv_n,
+                           (double2 *)g_fw.rho.p);
         hipLaunchKernelGGL(k_force_local, dim3(natom), dim3(256), 0, s, (const double2 *)g_fw.rho.p,
                            (const int *)g_fw.qv.p, (const double *)g_fw.fq.p, nq,
                            (const double *)g_fw.rat.p, (const int *)g_fw.species.p, (double *)g_fw.fl.p);
@@ -2059,7 +2064,7 @@ int cpwgpu_finalize(void) {
     for (DevBuf *b : {&g_dt.xa, &g_dt.hxa, &g_dt.xb, &g_dt.hxb, &g_dt.pb, &g_dt.cpk, &g_dt.hd, &g_dt.m,
                       &g_dt.g, &g_dt.g0,
                       &g_dt.pm, &g_dt.w, &g_dt.e, &g_dt.info, &g_dt.res, &g_dt.act, &g_dt.d, &g_dt.mones,
-                      &g_dt.eall, &g_fw.rhor})
+                      &g_dt.eall, &g_fw.rhor, &g_fw.t2, &g_fw.t2s})
         b->release();
     if (g_aux) {
         HIPCHK(hipStreamSynchronize(g_aux));

@@ -242,25 +242,40 @@ __global__ __launch_bounds__(256) void k_rho_r_reduce(const double *__restrict__
     rho[q] = s;
 }
 
-// rho(q) for each listed q: (1/N) sum_grp sum_z T2 exp(-2 pi i q3 z / N3)
-__global__ void k_rho_q(const double2 *__restrict__ t2, const int *__restrict__ qv, int nq,
-                        const double2 *__restrict__ tw3, int n3, int ngrp, int Q1, int Q2,
-                        double inv_n, double2 *__restrict__ rho) {
+// T2s[z][q2][q1] = sum over band groups of T2[grp][z][q2][q1], in group order
+__global__ void k_t2_sum(const double2 *__restrict__ t2, long nplane, int ngrp, double2 *__restrict__ t2s) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nplane) return;
+    double sr = 0.0, si = 0.0;
+    for (int grp = 0; grp < ngrp; grp++) {
+        const double2 a = t2[grp * nplane + i];
+        sr += a.x;
+        si += a.y;
+    }
+    t2s[i] = make_double2(sr, si);
+}
+
+// rho(q) for each listed q: (1/N) sum_z T2s exp(-2 pi i q3 z / N3)
+__global__ void k_rho_q(const double2 *__restrict__ t2s, const int *__restrict__ qv, int nq,
+                        const double2 *__restrict__ tw3, int n3, int Q1, int Q2, double inv_n,
+                        double2 *__restrict__ rho) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nq) return;
     const int q1 = qv[3 * k], q2 = qv[3 * k + 1], q3 = qv[3 * k + 2];
     const int W1 = 2 * Q1 + 1, W2 = 2 * Q2 + 1;
     const long off = (long)(q2 + Q2) * W1 + (q1 + Q1);
+    // twiddle index e = q3 z mod n3, advanced by de per plane
+    const int de = ((q3 % n3) + n3) % n3;
+    int e = 0;
     double sr = 0.0, si = 0.0;
-    for (int grp = 0; grp < ngrp; grp++)
-        for (int z = 0; z < n3; z++) {
-            const double2 a = t2[((long)grp * n3 + z) * W2 * W1 + off];
-            int e = (q3 * z) % n3;
-            if (e < 0) e += n3;
-            const double2 w = tw3[e];
-            sr += a.x * w.x + a.y * w.y;
-            si += a.y * w.x - a.x * w.y;
-        }
+    for (int z = 0; z < n3; z++) {
+        const double2 a = t2s[(long)z * W2 * W1 + off];
+        const double2 w = tw3[e];
+        sr += a.x * w.x + a.y * w.y;
+        si += a.y * w.x - a.x * w.y;
+        e += de;
+        if (e >= n3) e -= n3;
+    }
     rho[k] = make_double2(sr * inv_n, si * inv_n);
 }
 
