@@ -1618,15 +1618,20 @@ struct ForceWs {
     DevBuf t2, t2s, rho, qv, fq, rat, species, patom, occ, fl, fnl, enl, p, rhor;
 } g_fw;
 
-// register shape of the density pass: x lanes, y lanes, threads, waves per SIMD
-constexpr int DEN_GX = 8, DEN_GY = 8, DEN_T = 256, DEN_W = 3;
-// built when the line split divides N and a lane's row accumulators (rows per
-// x task x N / GX doubles) stay within 27: spill-free at 3 waves per SIMD for
-// 24, 48, 64 and 72 (80, 96 and 128 spill; they keep k_density_fast)
-template <int N> constexpr bool den_reg_ok() {
-    return reg_ok<double, N, DEN_GX, DEN_GY>() &&
-           ((N + DEN_T / DEN_GX - 1) / (DEN_T / DEN_GX)) * (N / DEN_GX) <= 27;
+// register shape of the density pass: x lanes, y lanes, waves per SIMD
+constexpr int DEN_GX = 8, DEN_GY = 8, DEN_W = 3;
+// threads of the register density pass for line length N: the smallest of 256,
+// 320 and 384 that keeps a lane's row accumulators (rows per x task x N / GX
+// doubles) within 27, for lines of at most 10 values per lane: spill-free at 3
+// waves per SIMD (24-72: 256, 80: 320); 0 = not built (96 and 128 spill, 108 has
+// no 8-lane split: k_density_fast)
+template <int N> constexpr int den_threads() {
+    if (!reg_ok<double, N, DEN_GX, DEN_GY>() || N / DEN_GX > 10) return 0;
+    for (int t : {256, 320, 384})
+        if (((N + t / DEN_GX - 1) / (t / DEN_GX)) * (N / DEN_GX) <= 27) return t;
+    return 0;
 }
+template <int N> constexpr bool den_reg_ok() { return den_threads<N>() > 0; }
 
 template <int N>
 int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *occ, int nb, int bpg,
@@ -1635,16 +1640,17 @@ int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *o
     if constexpr (den_reg_ok<N>()) {
         if (rhor) {
             // the density planes alone: register lines, accumulation in registers
+            constexpr int DT = den_threads<N>();
             const int S = reg_stride(g.nx);
             const size_t dyn = 16 * ((size_t)N + (size_t)N * S);
             if (dyn > 160 * 1024) return fail(CPWGPU_ERR_SIZE, "density plane %d does not fit in LDS", N);
             static bool attr_r = false;
             if (!attr_r) {
-                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DEN_T, DEN_W>,
+                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                 attr_r = true;
             }
-            hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DEN_T, DEN_W>), dim3(g.n[2] * ngrp), dim3(DEN_T),
+            hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W>), dim3(g.n[2] * ngrp), dim3(DT),
                                dyn, s, colbuf, (const int *)p->col_uy.p, occ, (const double2 *)p->tw[0].p, g, nb,
                                bpg, S, rhor);
             HIPCHK(hipGetLastError());
@@ -1741,16 +1747,46 @@ int plan_forces(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, const 
         if ((rc = local_stage<double>(p, la, (cplx<double> *)p->colbuf.p, (const double *)p->v64.p,
                                       ST_COL_BWD, s)))
             return rc;
-        rc = CPWGPU_ERR_SIZE;
-#define CPW_DEN(N) if (g.n[0] == N) rc = launch_density<N>(p, (const cplx<double> *)p->colbuf.p, occ, neig, bpg, ngrp, Q1, Q2, (double2 *)g_fw.t2.p, s);
-        CPW_FAST_SIZES(CPW_DEN)
-#undef CPW_DEN
-        if (rc) return rc;
-        // the band groups first (fixed order, all of the GPU), then per q the z sum
-        const long nplane = (long)g.n[2] * (2 * Q2 + 1) * (2 * Q1 + 1);
+        bool reg = false;
+#define CPW_DR(N) if (g.n[0] == N) reg = den_reg_ok<N>();
+        CPW_FAST_SIZES(CPW_DR)
+#undef CPW_DR
+        const int W1 = 2 * Q1 + 1;
+        const long nplane = (long)g.n[2] * (2 * Q2 + 1) * W1;
         if ((rc = g_fw.t2s.reserve(16 * (size_t)nplane))) return rc;
-        hipLaunchKernelGGL(k_t2_sum, dim3((unsigned)((nplane + 255) / 256)), dim3(256), 0, s,
-                           (const double2 *)g_fw.t2.p, nplane, ngrp, (double2 *)g_fw.t2s.p);
+        rc = CPWGPU_ERR_SIZE;
+        if (reg) {
+            // density planes per band group (register lines), then per z-plane
+            // the group sum and the partial transforms along x and y
+            const long nmesh = (long)g.n[0] * g.n[1] * g.n[2];
+            const size_t lds = 16 * ((size_t)g.n[0] + g.n[1] + (size_t)g.n[1] * W1) + 8 * (size_t)g.n[0] * g.n[1];
+            if (lds > 160 * 1024) return fail(CPWGPU_ERR_SIZE, "density plane of %d x %d does not fit in LDS", g.n[0], g.n[1]);
+            if ((rc = g_fw.rhor.reserve(8 * (size_t)ngrp * nmesh))) return rc;
+            rc = CPWGPU_ERR_SIZE;
+#define CPW_DEN(N) if (g.n[0] == N) rc = launch_density<N>(p, (const cplx<double> *)p->colbuf.p, occ, neig, bpg, ngrp, 0, 0, nullptr, s, (double *)g_fw.rhor.p);
+            CPW_FAST_SIZES(CPW_DEN)
+#undef CPW_DEN
+            if (rc) return rc;
+            static bool attr = false;
+            if (!attr) {
+                HIPCHK(hipFuncSetAttribute((const void *)k_rho_partial, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024));
+                attr = true;
+            }
+            hipLaunchKernelGGL(k_rho_partial, dim3(g.n[2]), dim3(256), lds, s, (const double *)g_fw.rhor.p, ngrp,
+                               g.n[0], g.n[1], g.n[2], Q1, Q2, (const double2 *)p->tw[0].p,
+                               (const double2 *)p->tw[1].p, (double2 *)g_fw.t2s.p);
+        } else {
+            // partial transforms per band group inside the density pass, then the
+            // band groups summed (fixed order, all of the GPU)
+#define CPW_DEN(N) if (g.n[0] == N) rc = launch_density<N>(p, (const cplx<double> *)p->colbuf.p, occ, neig, bpg, ngrp, Q1, Q2, (double2 *)g_fw.t2.p, s);
+            CPW_FAST_SIZES(CPW_DEN)
+#undef CPW_DEN
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_t2_sum, dim3((unsigned)((nplane + 255) / 256)), dim3(256), 0, s,
+                               (const double2 *)g_fw.t2.p, nplane, ngrp, (double2 *)g_fw.t2s.p);
+        }
+        // per q the z sum
         hipLaunchKernelGGL(k_rho_q, dim3((nq + 255) / 256), dim3(256), 0, s, (const double2 *)g_fw.t2s.p,
                            (const int *)g_fw.qv.p, nq, (const double2 *)p->tw[2].p, g.n[2], Q1, Q2, p->inv_n,
                            (double2 *)g_fw.rho.p);

@@ -242,6 +242,64 @@ __global__ __launch_bounds__(256) void k_rho_r_reduce(const double *__restrict__
     rho[q] = s;
 }
 
+// Per z-plane (one block each): rho(x, y) = sum over band groups of
+// rhor[grp][z][y][x] in group order, then the partial transforms of
+// k_density_fast: T2[z][q2][q1] = sum_y e^{-2 pi i q2 y/n2} sum_x
+// e^{-2 pi i q1 x/n1} rho(x, y), |q1| <= Q1, |q2| <= Q2.  Dynamic LDS:
+// tw1[n1], tw2[n2] (exp(+2 pi i e/n)), T1[n2][2 Q1 + 1], rho[n2][n1].
+__global__ __launch_bounds__(256) void k_rho_partial(const double *__restrict__ rhor, int ngrp, int n1, int n2,
+                                                     int n3, int Q1, int Q2, const double2 *__restrict__ tw1g,
+                                                     const double2 *__restrict__ tw2g, double2 *__restrict__ t2) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int W1 = 2 * Q1 + 1, W2 = 2 * Q2 + 1;
+    double2 *tw1 = reinterpret_cast<double2 *>(smem_raw);
+    double2 *tw2 = tw1 + n1;
+    double2 *t1 = tw2 + n2;                                  // [n2][W1]
+    double *acc = reinterpret_cast<double *>(t1 + n2 * W1);  // [n2][n1]
+    const int z = blockIdx.x;
+    const long plane = (long)n1 * n2;
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) tw1[i] = tw1g[i];
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) tw2[i] = tw2g[i];
+    for (long i = threadIdx.x; i < plane; i += blockDim.x) {
+        double v = 0.0;
+        for (int grp = 0; grp < ngrp; grp++) v += rhor[((long)grp * n3 + z) * plane + i];
+        acc[i] = v;
+    }
+    __syncthreads();
+    // along x: T1[y][q1 + Q1] = sum_x rho(x, y) conj(tw1[q1 x mod n1])
+    for (int idx = threadIdx.x; idx < n2 * W1; idx += blockDim.x) {
+        const int y = idx / W1, q1 = idx - y * W1 - Q1;
+        const int de = ((q1 % n1) + n1) % n1;
+        int e = 0;
+        double sr = 0.0, si = 0.0;
+        for (int x = 0; x < n1; x++) {
+            const double a = acc[y * n1 + x];
+            sr += a * tw1[e].x;
+            si -= a * tw1[e].y;
+            e += de;
+            if (e >= n1) e -= n1;
+        }
+        t1[idx] = make_double2(sr, si);
+    }
+    __syncthreads();
+    // along y: T2[z][q2 + Q2][q1 + Q1] = sum_y T1[y][q1] conj(tw2[q2 y mod n2])
+    double2 *out = t2 + (long)z * W2 * W1;
+    for (int idx = threadIdx.x; idx < W2 * W1; idx += blockDim.x) {
+        const int q2 = idx / W1 - Q2, q1i = idx - (idx / W1) * W1;
+        const int de = ((q2 % n2) + n2) % n2;
+        int e = 0;
+        double sr = 0.0, si = 0.0;
+        for (int y = 0; y < n2; y++) {
+            const double2 a = t1[y * W1 + q1i], w = tw2[e];
+            sr += a.x * w.x + a.y * w.y;
+            si += a.y * w.x - a.x * w.y;
+            e += de;
+            if (e >= n2) e -= n2;
+        }
+        out[idx] = make_double2(sr, si);
+    }
+}
+
 // T2s[z][q2][q1] = sum over band groups of T2[grp][z][q2][q1], in group order
 __global__ void k_t2_sum(const double2 *__restrict__ t2, long nplane, int ngrp, double2 *__restrict__ t2s) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -5,9 +5,9 @@ psi_n(r) the unnormalised G -> r transform.
 Checked against numpy's inverse FFT of psi scattered onto the mesh (x N, the
 reference's unnormalised sign convention) to 1e-12 relative, on the 2-atom
 cell (24^3, with and without id > n1 padding, which must come back zero, and
-with the box enlarged to 80^3) and the 64-atom cell (72^3); Parseval
-sum_r rho = N sum_n occ_n ||psi_n||^2 to 1e-12.  24^3 and 72^3 run the
-register-line density kernel, 80^3 the LDS one."""
+with the box enlarged to 80^3 and 96^3) and the 64-atom cell (72^3); Parseval
+sum_r rho = N sum_n occ_n ||psi_n||^2 to 1e-12.  24^3 to 80^3 run the
+register-line density kernel, 96^3 the LDS one."""
 
 import numpy as np
 import pytest
@@ -45,10 +45,10 @@ def _numpy_density(d, psi, occ):
     return rho
 
 
-# (workload, id - n1, bands, kmscr(1:3) override): 24^3 and 72^3 run the
-# register-line density kernel, 80^3 (the 2-atom cell's box enlarged) the LDS one
+# (workload, id - n1, bands, kmscr(1:3) override): 24^3, 72^3 and 80^3 (the
+# 2-atom cell's box enlarged) run the register-line density kernel, 96^3 the LDS one
 @pytest.mark.parametrize("cfg,idshift,nb,kmax", [(0, 0, 16, None), (0, 3, 9, None), (1, 0, 12, None),
-                                                 (0, 0, 10, 39)])
+                                                 (0, 0, 10, 39), (0, 0, 6, 47)])
 def test_density_matches_numpy_fft(torch_dev, hk, cfg, idshift, nb, kmax):
     torch = torch_dev
     d = synth.make_config(cfg, nbands=nb, idshift=idshift,

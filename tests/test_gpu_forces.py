@@ -33,11 +33,14 @@ def _gpu_forces(torch, d):
     return f, enl
 
 
-@pytest.mark.parametrize("cfg", ["si2f", 4])
-def test_forces_match_oracle(torch_dev, cfg):
+# (workload, kmscr(1:3) override): 24^3 and 80^3 run the register-line density
+# pass with per-plane partial transforms, 96^3 (the 2-atom box enlarged) the LDS
+# density pass with per-group partial transforms
+@pytest.mark.parametrize("cfg,kmax", [("si2f", None), (4, None), ("si2f", 47)])
+def test_forces_match_oracle(torch_dev, cfg, kmax):
     from oracle import forces as Fo
 
-    d = synth.make_config(cfg)
+    d = synth.make_config(cfg, kmax_override=None if kmax is None else [kmax] * 3)
     f, enl = _gpu_forces(torch_dev, d)
     fo, enlo = Fo.forces(d, d["psi"])
     assert np.abs(fo).max() > 1e-4
