@@ -1,7 +1,8 @@
 """Build recipe for the native libraries of cpw2000_amd (in-tree).
 
   libcpwgpu.so          HIP kernels + C ABI (include/cpwgpu.h), gfx950
-  libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16, hk_psi_spin_c16 and ditsp_c16 over the C ABI (flang)
+  libcpwgpu_fortran.so  Fortran drop-ins hk_psi_c16, hk_psi_spin_c16 and ditsp_c16, and the all-k
+                        hk_psi_allk_c16, over the C ABI (flang)
   libcpwprobe.so        VALU issue-rate probe (measurement only: ``bench.py --issue-rates``)
 
 Run ``python -m cpw2000_amd.build``.
@@ -56,7 +57,7 @@ def build_probe():
 
 def build_fortran():
     srcs = [os.path.join(PKG, "fortran", f)
-            for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90")]
+            for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90", "hk_psi_allk_c16.f90")]
     tmp = FLIB + ".tmp"
     objdir = os.path.join(PKG, "fortran")
     _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, *srcs,

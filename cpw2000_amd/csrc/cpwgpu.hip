@@ -1023,18 +1023,30 @@ int apply_spin(cpwgpu_plan_s *p, int neig, const double2 *psi, long ldp, double2
 }
 
 // ---- drop-in state --------------------------------------------------------
-struct DropIn {
+// a plan kept between host-array calls while the plane-wave set is unchanged
+struct PlanSlot {
     cpwgpu_plan_s *plan = nullptr;
     int key_ng = -1, key_mtxd = -1;
     int key_kmscr[7] = {0};
+    void clear() {
+        delete plan;
+        plan = nullptr;
+        key_mtxd = -1;
+    }
+};
+
+struct DropIn {
+    PlanSlot slot;
     DevBuf psi, hpsi, hs, hd, c8;
 } g_drop;
 
-int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
-                 const int *kmscr, int mxddim, int mxdgve, cpwgpu_plan_s **out) {
-    cpwgpu_plan_s *p = g_drop.plan;
-    bool same = p && g_drop.key_mtxd == mtxd && g_drop.key_ng == ng &&
-                std::equal(kmscr, kmscr + 7, g_drop.key_kmscr);
+// the slot's plan if it holds the same plane waves in the same order on the
+// same mesh (ekpg refreshed when it changed), else a new plan in the slot
+int slot_plan(PlanSlot &slot, int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
+              const int *kmscr, int mxddim, int mxdgve, cpwgpu_plan_s **out) {
+    cpwgpu_plan_s *p = slot.plan;
+    bool same = p && slot.key_mtxd == mtxd && slot.key_ng == ng &&
+                std::equal(kmscr, kmscr + 7, slot.key_kmscr);
     if (same) {
         // same plane waves in the same order on the same mesh?
         std::vector<int> kx, ky, kz;
@@ -1048,18 +1060,17 @@ int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int
             same = p->key_pts[i] == ((long)kz[i] * ns[1] + ky[i]) * kmscr[6] + kx[i];
     }
     if (!same) {
-        delete g_drop.plan;
-        g_drop.plan = nullptr;
+        slot.clear();
         p = new cpwgpu_plan_s();
         int st = build_plan(p, mtxd, ng, kgv, ekpg, isort, kmscr, mxddim, mxdgve);
         if (st) {
             delete p;
             return st;
         }
-        g_drop.plan = p;
-        g_drop.key_mtxd = mtxd;
-        g_drop.key_ng = ng;
-        std::copy(kmscr, kmscr + 7, g_drop.key_kmscr);
+        slot.plan = p;
+        slot.key_mtxd = mtxd;
+        slot.key_ng = ng;
+        std::copy(kmscr, kmscr + 7, slot.key_kmscr);
     } else if (!std::equal(ekpg, ekpg + mtxd, p->h_ekpg.begin())) {
         p->h_ekpg.assign(ekpg, ekpg + mtxd);
         int st = upload(p->ekpg, ekpg, mtxd);
@@ -1068,6 +1079,17 @@ int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int
     *out = p;
     return CPWGPU_OK;
 }
+
+int drop_in_plan(int mtxd, int ng, const int *kgv, const double *ekpg, const int *isort,
+                 const int *kmscr, int mxddim, int mxdgve, cpwgpu_plan_s **out) {
+    return slot_plan(g_drop.slot, mtxd, ng, kgv, ekpg, isort, kmscr, mxddim, mxdgve, out);
+}
+
+// all-k host entry: one plan slot per k-point, device copies of psi / H psi
+struct AllK {
+    std::vector<PlanSlot> slots;
+    DevBuf psi, hpsi;
+} g_allk;
 
 }  // namespace
 
@@ -2083,9 +2105,11 @@ int cpwgpu_init(int device) {
 
 int cpwgpu_finalize(void) {
     std::lock_guard<std::mutex> lk(g_mu);
-    delete g_drop.plan;
-    g_drop.plan = nullptr;
-    g_drop.key_mtxd = -1;
+    g_drop.slot.clear();
+    for (PlanSlot &sl : g_allk.slots) sl.clear();
+    g_allk.slots.clear();
+    g_allk.psi.release();
+    g_allk.hpsi.release();
     g_drop.psi.release();
     g_drop.hpsi.release();
     g_drop.hs.release();
@@ -2486,6 +2510,64 @@ int cpwgpu_hk_psi_c16(const int *mtxd, const int *neig, const double *psi, doubl
     if (st) return st;
     HIPCHK(hipMemcpy2DAsync(hpsi, 16 * (size_t)*mxddim, g_drop.hpsi.p, colb, colb, *neig,
                             hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_hk_psi_allk_c16(const int *nrk, const int *mtxd_allk, const int *nband, const double *psi_allk,
+                           double *hpsi_allk, const int *ng, const int *kgv, const double *ekpg_allk,
+                           const int *isort_allk, const double *vscr, const int *kmscr, const double *anlga_allk,
+                           const double *xnlkb_allk, const int *nanl_allk, const int *mxddim, const int *mxdbnd,
+                           const int *mxdanl, const int *mxdgve, const int *mxdscr, const int *mxdnrk) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!nrk || !mtxd_allk || !nband || !ng || !kmscr || !nanl_allk || !mxddim || !mxdbnd || !mxdanl ||
+        !mxdgve || !mxdscr || !mxdnrk)
+        return fail(CPWGPU_ERR_ARG, "null scalar argument");
+    if (*nrk < 0 || *nrk > *mxdnrk) return fail(CPWGPU_ERR_ARG, "nrk = %d, mxdnrk = %d", *nrk, *mxdnrk);
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", *ng, *mxdgve);
+    int ns[3], mfft, mwrk;
+    size_fft(kmscr, ns, &mfft, &mwrk);
+    if (mwrk > *mxdscr)
+        return fail(CPWGPU_ERR_MXDSCR, "mxdwrk = %d is greater than mxdscr = %d", mwrk, *mxdscr);
+    size_t maxcol = 0;
+    for (int k = 0; k < *nrk; k++) {
+        if (mtxd_allk[k] < 0 || mtxd_allk[k] > *mxddim)
+            return fail(CPWGPU_ERR_MTXD, "k-point %d: mtxd = %d, mxddim = %d", k + 1, mtxd_allk[k], *mxddim);
+        if (nband[k] < 0 || nband[k] > *mxdbnd)
+            return fail(CPWGPU_ERR_ARG, "k-point %d: nband = %d, mxdbnd = %d", k + 1, nband[k], *mxdbnd);
+        if (nanl_allk[k] < 0 || nanl_allk[k] > *mxdanl)
+            return fail(CPWGPU_ERR_ARG, "k-point %d: nanl = %d > mxdanl = %d", k + 1, nanl_allk[k], *mxdanl);
+        maxcol = std::max(maxcol, 16 * (size_t)mtxd_allk[k] * nband[k]);
+    }
+    if (*nrk == 0 || maxcol == 0) return CPWGPU_OK;
+    if (!psi_allk || !hpsi_allk || !vscr || !kgv || !ekpg_allk || !isort_allk)
+        return fail(CPWGPU_ERR_ARG, "null array argument");
+    int st;
+    if (g_allk.slots.size() < (size_t)*nrk) g_allk.slots.resize(*nrk);
+    const size_t md = (size_t)*mxddim, mb = (size_t)*mxdbnd, ma = (size_t)*mxdanl;
+    for (int k = 0; k < *nrk; k++) {
+        const int mtxd = mtxd_allk[k], nb = nband[k], nanl = nanl_allk[k];
+        cpwgpu_plan_s *p = nullptr;
+        if ((st = slot_plan(g_allk.slots[k], mtxd, *ng, kgv, ekpg_allk + k * md, isort_allk + k * md, kmscr,
+                            *mxddim, *mxdgve, &p)))
+            return st;
+        if (nb == 0 || mtxd == 0) continue;
+        if ((st = set_potential(p, vscr, *mxdscr))) return st;
+        if ((st = g_allk.psi.reserve(maxcol)) || (st = g_allk.hpsi.reserve(maxcol))) return st;
+        if ((st = set_projectors_host(p, anlga_allk ? anlga_allk + 2 * k * md * ma : nullptr,
+                                      xnlkb_allk ? xnlkb_allk + k * ma : nullptr, nanl, *mxddim)))
+            return st;
+        const size_t colb = 16 * (size_t)mtxd, off = 2 * k * md * mb;  // doubles
+        HIPCHK(hipMemcpy2DAsync(g_allk.psi.p, colb, psi_allk + off, 16 * md, colb, nb, hipMemcpyHostToDevice,
+                                g_stream));
+        const double *dpsi = (const double *)g_allk.psi.p;
+        double *dhpsi = (double *)g_allk.hpsi.p;
+        st = g_prec == CPWGPU_PREC_FP32 ? apply_t<float>(p, nb, dpsi, mtxd, dhpsi, mtxd, g_stream)
+                                        : apply_t<double>(p, nb, dpsi, mtxd, dhpsi, mtxd, g_stream);
+        if (st) return st;
+        HIPCHK(hipMemcpy2DAsync(hpsi_allk + off, 16 * md, g_allk.hpsi.p, colb, colb, nb, hipMemcpyDeviceToHost,
+                                g_stream));
+    }
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
 }

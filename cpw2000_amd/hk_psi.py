@@ -50,7 +50,7 @@ EXPORTS = (
     "cpwgpu_plan_set_variant", "cpwgpu_plan_get_variant", "cpwgpu_plan_tune", "cpwgpu_plan_tune_spin",
     "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
     "cpwgpu_hk_psi_spin_c16", "cpwgpu_plan_set_potential_spin", "cpwgpu_plan_set_projectors_spin",
-    "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8",
+    "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8", "cpwgpu_hk_psi_allk_c16",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
@@ -161,6 +161,31 @@ def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr
         _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr))
     _check(st)
     return hpsi
+
+
+def hk_psi_allk_c16(nrk, mtxd_allk, nband, psi_allk, hpsi_allk, ng, kgv, ekpg_allk, isort_allk, vscr,
+                    kmscr, anlga_allk, xnlkb_allk, nanl_allk, mxddim, mxdbnd, mxdanl, mxdgve, mxdscr, mxdnrk):
+    """H|psi> for every k-point of a self-consistency step (cpwgpu_hk_psi_allk_c16),
+    with the all-k arrays of the k-point loop of Src/cpw/cpw_scf.f90:488-544:
+    psi_allk/hpsi_allk complex128 (mxddim, mxdbnd, mxdnrk), isort_allk int32 and
+    ekpg_allk float64 (mxddim, mxdnrk), anlga_allk complex128 (mxddim, mxdanl,
+    mxdnrk), xnlkb_allk float64 (mxdanl, mxdnrk), mtxd_allk/nband/nanl_allk int32
+    (mxdnrk,); numpy, Fortran layout.  hpsi_allk is written in place (rows
+    0..mtxd_allk[k]-1 of the first nband[k] columns of each k-point)."""
+    for name, arr, dt in (("mtxd_allk", mtxd_allk, np.int32), ("nband", nband, np.int32),
+                          ("psi_allk", psi_allk, np.complex128), ("hpsi_allk", hpsi_allk, np.complex128),
+                          ("kgv", kgv, np.int32), ("ekpg_allk", ekpg_allk, np.float64),
+                          ("isort_allk", isort_allk, np.int32), ("vscr", vscr, np.float64),
+                          ("kmscr", kmscr, np.int32), ("anlga_allk", anlga_allk, np.complex128),
+                          ("xnlkb_allk", xnlkb_allk, np.float64), ("nanl_allk", nanl_allk, np.int32)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    _check(lib().cpwgpu_hk_psi_allk_c16(
+        _iref(nrk), _p(mtxd_allk), _p(nband), _p(psi_allk), _p(hpsi_allk), _iref(ng), _p(kgv),
+        _p(ekpg_allk), _p(isort_allk), _p(vscr), _p(kmscr), _p(anlga_allk), _p(xnlkb_allk),
+        _p(nanl_allk), _iref(mxddim), _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr),
+        _iref(mxdnrk)))
+    return hpsi_allk
 
 
 def hk_psi_c8(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr, anlga, xnlkb,

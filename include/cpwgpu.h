@@ -195,6 +195,27 @@ int cpwgpu_plan_tune(cpwgpu_plan_t plan, int neig, const double *psi_dev, long l
 int cpwgpu_plan_tune_spin(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp, double *hpsi_dev,
                           long ldh, int prec, void *stream);
 
+/* ---- every k-point of a self-consistency step in one call (host arrays) ------ */
+
+/* H|psi> for the nrk k-points of the irk loop of Src/cpw/cpw_scf.f90:488-544,
+ * with its all-k arrays (all by reference, Fortran layout):
+ *   psi_allk, hpsi_allk (mxddim, mxdbnd, mxdnrk) complex(8); k-point k holds
+ *     nband(k) bands of mtxd_allk(k) plane waves (ragged)
+ *   isort_allk (mxddim, mxdnrk), ekpg_allk (mxddim, mxdnrk)
+ *   anlga_allk (mxddim, mxdanl, mxdnrk) complex(8), xnlkb_allk (mxdanl, mxdnrk),
+ *     nanl_allk (mxdnrk): the projectors of each k-point
+ *   ng, kgv, vscr (mxdscr), kmscr(7): shared by all k-points, as in hk_psi_c16.
+ * Rows beyond mtxd_allk(k) and columns beyond nband(k) of hpsi_allk are not
+ * written.  One plan per k-point is kept between calls while its plane-wave
+ * set is unchanged; psi, V and the projectors cross PCIe on every call (the
+ * device-resident batch is cpwgpu_graph_*).  Precision: cpwgpu_set_precision.
+ * Errors: as cpwgpu_hk_psi_c16, per k-point. */
+int cpwgpu_hk_psi_allk_c16(const int *nrk, const int *mtxd_allk, const int *nband, const double *psi_allk,
+                           double *hpsi_allk, const int *ng, const int *kgv, const double *ekpg_allk,
+                           const int *isort_allk, const double *vscr, const int *kmscr, const double *anlga_allk,
+                           const double *xnlkb_allk, const int *nanl_allk, const int *mxddim, const int *mxdbnd,
+                           const int *mxdanl, const int *mxdgve, const int *mxdscr, const int *mxdnrk);
+
 /* ---- several k-points in one graph launch -------------------------------------- */
 
 typedef struct cpwgpu_graph_s *cpwgpu_graph_t;

@@ -81,6 +81,39 @@ def test_fortran_shim_exports_hk_psi_c16_and_ditsp_c16():
     assert re.search(r"\bT hk_psi_c16_\b", out)
     assert re.search(r"\bT ditsp_c16_\b", out)
     assert re.search(r"\bT hk_psi_spin_c16_\b", out)
+    assert re.search(r"\bT hk_psi_allk_c16_\b", out)
+
+
+def test_allk_argument_checks_before_any_device_work():
+    """cpwgpu_hk_psi_allk_c16 checks every k-point's sizes (and the shared mesh)
+    before it touches the device; nrk = 0 is a no-op."""
+    from cpw2000_amd import hk_psi
+
+    d = synth.make_config(0, nbands=2)
+    m, nanl = d["mtxd"], d["nanl"]
+    f = lambda shape, dt: np.zeros(shape, dt, order="F")
+
+    def args(**over):
+        a = dict(nrk=1, mtxd_allk=np.array([m, 0], np.int32), nband=np.array([2, 0], np.int32),
+                 psi_allk=f((m, 2, 2), np.complex128), hpsi_allk=f((m, 2, 2), np.complex128), ng=d["ng"],
+                 kgv=np.asfortranarray(d["kgv"], np.int32), ekpg_allk=f((m, 2), np.float64),
+                 isort_allk=f((m, 2), np.int32), vscr=np.asfortranarray(d["vscr"]),
+                 kmscr=np.asfortranarray(d["kmscr"], np.int32), anlga_allk=f((m, nanl, 2), np.complex128),
+                 xnlkb_allk=f((nanl, 2), np.float64), nanl_allk=np.array([nanl, 0], np.int32), mxddim=m,
+                 mxdbnd=2, mxdanl=nanl, mxdgve=d["mxdgve"], mxdscr=d["mxdscr"], mxdnrk=2)
+        a.update(over)
+        return a
+
+    hk_psi.hk_psi_allk_c16(**args(nrk=0))
+    for over, code in ((dict(nrk=3), hk_psi.ERR_ARG),
+                       (dict(mtxd_allk=np.array([m + 1, 0], np.int32)), hk_psi.ERR_MTXD),
+                       (dict(nband=np.array([3, 0], np.int32)), hk_psi.ERR_ARG),
+                       (dict(nanl_allk=np.array([nanl + 1, 0], np.int32)), hk_psi.ERR_ARG),
+                       (dict(mxdgve=d["ng"] - 1), hk_psi.ERR_NG),
+                       (dict(mxdscr=100), hk_psi.ERR_MXDSCR)):
+        with pytest.raises(hk_psi.CpwGpuError) as e:
+            hk_psi.hk_psi_allk_c16(**args(**over))
+        assert e.value.code == code, over
 
 
 def _ditsp_args(d, neig, **over):
