@@ -861,10 +861,12 @@ int set_projectors_spin_host(cpwgpu_plan_s *p, const double *anlsp, const double
 }
 
 // register shape of the coupled two-component plane kernel: (x lanes, y
-// lanes, threads, waves per SIMD); both components' rows live in registers
+// lanes, threads, waves per SIMD); both components' rows live in registers.
+// FP64: the two sub-planes (about 86 KB at 72) allow one block per CU, so the
+// block has 512 threads for 2 waves per SIMD (184 VGPRs fit that budget).
 template <typename R> struct SpinShape;
 template <> struct SpinShape<float> { static constexpr int GX = 8, GY = 8, T = 256, W = 3; };
-template <> struct SpinShape<double> { static constexpr int GX = 8, GY = 8, T = 256, W = 2; };
+template <> struct SpinShape<double> { static constexpr int GX = 8, GY = 8, T = 512, W = 2; };
 template <typename R, int N> constexpr bool spin_reg_ok() {
     return reg_ok<R, N, SpinShape<R>::GX, SpinShape<R>::GY>();
 }
