@@ -516,6 +516,9 @@ def main():
     ap.add_argument("--variant-sweep", action="store_true",
                     help="only time every projection / back-projection kernel alone (one "
                          "stream), then the calibration copy; prints their JSON and exits")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 64-atom headline (tuning, accuracy, timed loops, rooflines): "
+                         "for profiling its kernels")
     ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints"),
                     help="run only this section (after tuning the 64-atom plan) and print its JSON; "
                          "for profiling one path at a time")
@@ -664,13 +667,16 @@ def main():
         dst = src.clone()  # 512 MiB read + 512 MiB written
         torch.cuda.synchronize()
         del src, dst
+    if args.headline_only:
+        args.no_kpoints = args.no_ditsp = args.no_cpu = True
     if not args.no_kpoints:
         log("bench: 16 k-points, eager loop vs one graph launch")
         out["kpoints_graph"] = bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))
-    log("bench: two-component bands and the density")
-    out["spinor_density"] = bench_spin_density(torch, hk, synth, np, d, max(5, args.steps // 3))
-    log("bench: atomic forces (Si/Ge 64 atoms, 128 bands)")
-    out["forces"] = bench_forces(torch, hk, synth, np, 5)
+    if not args.headline_only:
+        log("bench: two-component bands and the density")
+        out["spinor_density"] = bench_spin_density(torch, hk, synth, np, d, max(5, args.steps // 3))
+        log("bench: atomic forces (Si/Ge 64 atoms, 128 bands)")
+        out["forces"] = bench_forces(torch, hk, synth, np, 5)
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
@@ -700,17 +706,18 @@ def main():
             iss["measured_alone_us"] = alone
             iss["frac_alone"] = round(iss["floor_us"] / alone, 4)
         out[key]["issue"] = iss
-    # the Fortran-facing drop-in: host arrays in and out (PCIe included)
-    hk.set_precision(hk.PREC_FP64)
-    hpsi_h = np.zeros_like(d["psi"], order="F")
-    hk.hk_psi_c16_dict(d, hpsi=hpsi_h)
-    t0 = time.perf_counter()
-    ncall = 3
-    for _ in range(ncall):
+    if not args.headline_only:
+        # the Fortran-facing drop-in: host arrays in and out (PCIe included)
+        hk.set_precision(hk.PREC_FP64)
+        hpsi_h = np.zeros_like(d["psi"], order="F")
         hk.hk_psi_c16_dict(d, hpsi=hpsi_h)
-    dt = (time.perf_counter() - t0) / ncall
-    out["dropin_host_fp64"] = {"bands_per_s": round(nb / dt, 1), "ms_per_call": round(dt * 1e3, 2),
-                               "note": "psi/H psi/V/projectors copied host<->device every call"}
+        t0 = time.perf_counter()
+        ncall = 3
+        for _ in range(ncall):
+            hk.hk_psi_c16_dict(d, hpsi=hpsi_h)
+        dt = (time.perf_counter() - t0) / ncall
+        out["dropin_host_fp64"] = {"bands_per_s": round(nb / dt, 1), "ms_per_call": round(dt * 1e3, 2),
+                                   "note": "psi/H psi/V/projectors copied host<->device every call"}
     if not args.no_cpu:
         log(f"bench: CPU baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(d, args.cpu_seconds)
