@@ -366,6 +366,41 @@ def bench_forces(torch, hk, synth, np, steps):
             "max_force_ha_bohr": round(float(np.abs(f).max()), 6)}
 
 
+def bench_projectors(torch, hk, synth, np, d, steps):
+    """KB projectors built on the device (cpwgpu_plan_build_projectors_kb, the
+    construction of Src/omp/proj_nl_kb_c16.f90) for the 64-atom Si cell: 256
+    s and p projectors on 23,847 plane waves from interpolation tables of
+    4,242 points per (species, l).  Kernel time from HIP events around each
+    launch; per-call time includes the table upload and the synchronisation.
+    Algorithmic HBM bytes per launch: the projectors written once (16 B per
+    element) plus g of each plane wave read once (24 B)."""
+    spec = synth.config_spec(1)
+    pp = synth.kb_pseudo(spec)
+    plan = hk.Plan.from_inputs(d, projectors=False)
+    args = (np.zeros(3), pp["nanl"], pp["nqnl"], pp["delqnl"], pp["vkb"], pp["nkb"], pp["ntype"], pp["natom"],
+            pp["rat"], pp["adot"])
+    for _ in range(2):
+        plan.build_projectors_kb(*args)
+    hk.profile_read()
+    hk.profile(True, ["k_proj_kb"])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.build_projectors_kb(*args)
+    ms_call = (time.perf_counter() - t0) / steps * 1e3
+    hk.profile(False)
+    ms, cnt = hk.profile_read()["k_proj_kb"]
+    plan.close()
+    mtxd, nanl = int(d["mtxd"]), int(pp["nanl"])
+    us = ms / max(cnt, 1) * 1e3
+    nbytes = 16 * mtxd * nanl + 24 * mtxd
+    gbs = nbytes / (us * 1e-6) / 1e9
+    return {"workload": "Si64 L=20.52, Ecut 15 Ha: 256 s+p KB projectors on 23,847 plane waves",
+            "nanl": nanl, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
+            "us_per_kernel": round(us, 2), "ms_per_call": round(ms_call, 3),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(gbs / PEAK_HBM_GBS, 3), "bytes_per_launch": nbytes}}
+
+
 def bench_cfg2(torch, hk, synth, np, steps):
     """216-atom Si (108^3 mesh, 864 projectors, 512 bands): throughput of both
     paths and the FP32-mixed error against FP64 (H psi norm and Ritz values of
@@ -519,7 +554,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the 64-atom headline (tuning, accuracy, timed loops, rooflines): "
                          "for profiling its kernels")
-    ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints"),
+    ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints", "projectors"),
                     help="run only this section (after tuning the 64-atom plan) and print its JSON; "
                          "for profiling one path at a time")
     ap.add_argument("--issue-rates", action="store_true",
@@ -561,11 +596,12 @@ def main():
 
     # autotuner: plane-stage kernel and split-K target per precision, timed on
     # this workload (part of the warm-up; every later apply uses the choice)
-    if args.only in ("spinor", "forces", "kpoints"):
+    if args.only in ("spinor", "forces", "kpoints", "projectors"):
         # these sections build and tune their own plans
         steps = max(5, args.steps // 3)
         sec = {"spinor": lambda: bench_spin_density(torch, hk, synth, np, d, steps),
                "forces": lambda: bench_forces(torch, hk, synth, np, 5),
+               "projectors": lambda: bench_projectors(torch, hk, synth, np, d, max(10, args.steps)),
                "kpoints": lambda: bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))}[args.only]
         print(json.dumps({args.only: sec()}), flush=True)
         plan.close()
@@ -677,6 +713,8 @@ def main():
         out["spinor_density"] = bench_spin_density(torch, hk, synth, np, d, max(5, args.steps // 3))
         log("bench: atomic forces (Si/Ge 64 atoms, 128 bands)")
         out["forces"] = bench_forces(torch, hk, synth, np, 5)
+        log("bench: KB projectors built on the device (64-atom Si)")
+        out["projectors_kb"] = bench_projectors(torch, hk, synth, np, d, 20)
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)

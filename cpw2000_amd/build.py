@@ -57,7 +57,8 @@ def build_probe():
 
 def build_fortran():
     srcs = [os.path.join(PKG, "fortran", f)
-            for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90", "hk_psi_allk_c16.f90")]
+            for f in ("hk_psi_c16.f90", "hk_psi_spin_c16.f90", "ditsp_c16.f90", "hk_psi_allk_c16.f90",
+                      "proj_nl_kb_c16.f90")]
     tmp = FLIB + ".tmp"
     objdir = os.path.join(PKG, "fortran")
     _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", objdir, "-o", tmp, *srcs,

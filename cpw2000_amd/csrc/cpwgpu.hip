@@ -28,6 +28,7 @@
 #include "forces.hpp"
 #include "ditsp.hpp"
 #include "spin.hpp"
+#include "proj_kb.hpp"
 
 using namespace cpwgpu;
 
@@ -812,6 +813,127 @@ int set_projectors_host(cpwgpu_plan_s *p, const double *anlga, const double *xnl
         HIPCHK(hipMemcpy2DAsync(p->anl64.p, 16 * (size_t)p->lda, anlga, 16 * (size_t)mxddim,
                                 16 * (size_t)p->mtxd, nanl, hipMemcpyHostToDevice, g_stream));
     if ((st = upload(p->xnl, xnlkb, nanl))) return st;
+    return CPWGPU_OK;
+}
+
+// ---- KB projectors on the device (Src/omp/proj_nl_kb_c16.f90:23-275) --------------
+DevBuf g_kb_tab, g_kb_types, g_kb_atoms, g_kb_gw, g_kb_out;
+
+struct KbSetup {
+    KbCell cell{};
+    std::vector<KbType> types;
+    std::vector<KbAtom> atoms;   // atoms of the types that have projectors
+    std::vector<double> tab;     // vkb(-2:mxdlqp, l, 0, type) lines, packed per (type, l)
+    std::vector<double> xnl;     // nkb of each column
+    long vstride = 0;
+};
+
+// Host half of proj_nl_kb_c16: checks (:143-157, and the cell of
+// Src/crystal/adot_to_avec.f90 / adot_to_bdot.f90), column bookkeeping
+// (:169-186), packed tables and xnlkb (:265-267).  Arrays in the reference's
+// layouts: vkb(-2:mxdlqp, 0:3, -1:1, mxdtyp), nkb(0:3, -1:1, mxdtyp),
+// rat(3, mxdatm, mxdtyp), adot(3, 3).
+int kb_setup(const double *rkpt, int nanl, const int *nqnl, const double *delqnl, const double *vkb,
+             const int *nkb, int ntype, const int *natom, const double *rat, const double *adot, int mxdtyp,
+             int mxdatm, int mxdlqp, KbSetup &s) {
+    if (!rkpt || !adot) return fail(CPWGPU_ERR_ARG, "null rkpt/adot");
+    if (ntype < 0 || ntype > mxdtyp) return fail(CPWGPU_ERR_ARG, "ntype = %d, mxdtyp = %d", ntype, mxdtyp);
+    if (mxdlqp < 0 || mxdatm < 0) return fail(CPWGPU_ERR_ARG, "mxdlqp = %d, mxdatm = %d", mxdlqp, mxdatm);
+    if (ntype > 0 && (!nqnl || !delqnl || !vkb || !nkb || !natom || !rat))
+        return fail(CPWGPU_ERR_ARG, "null pseudopotential array");
+    // avec upper triangular with a1 along x and a2 in the xy plane: adot = avec^T avec
+    const double eps = 1e-10;
+    double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};   // a[r][c] = avec(r+1, c+1)
+    const auto A = [&](int r, int c) { return adot[r + 3 * c]; };
+    if (!(A(0, 0) >= eps)) return fail(CPWGPU_ERR_ARG, "adot_to_avec: avec(1,1) = %g", A(0, 0));
+    a[0][0] = std::sqrt(A(0, 0));
+    a[0][1] = A(0, 1) / a[0][0];
+    double t = A(1, 1) - a[0][1] * a[0][1];
+    if (!(t >= eps)) return fail(CPWGPU_ERR_ARG, "adot_to_avec: avec(2,2) = %g", t);
+    a[1][1] = std::sqrt(t);
+    a[0][2] = A(0, 2) / a[0][0];
+    a[1][2] = (A(1, 2) - a[0][1] * a[0][2]) / a[1][1];
+    t = A(2, 2) - a[1][2] * a[1][2] - a[0][2] * a[0][2];
+    if (!(t >= eps)) return fail(CPWGPU_ERR_ARG, "adot_to_avec: avec(3,3) = %g", t);
+    a[2][2] = std::sqrt(t);
+    // bvec = 2 pi inv(avec)^T; inv of the upper triangular avec by back substitution
+    double ia[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int c = 0; c < 3; c++) {
+        ia[c][c] = 1.0 / a[c][c];
+        for (int r = c - 1; r >= 0; r--) {
+            double acc = 0.0;
+            for (int k = r + 1; k <= c; k++) acc += a[r][k] * ia[k][c];
+            ia[r][c] = -acc / a[r][r];
+        }
+    }
+    const double twopi = 6.283185307179586477;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) s.cell.b[r + 3 * c] = twopi * ia[c][r];
+    // vcell^2 = det adot
+    const double det = A(0, 0) * (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) -
+                       A(0, 1) * (A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0)) +
+                       A(0, 2) * (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0));
+    if (!(det >= eps)) return fail(CPWGPU_ERR_ARG, "adot_to_bdot: vcell squared = %g", det);
+    s.cell.fac = 1.0 / std::sqrt(std::sqrt(det));
+    for (int j = 0; j < 3; j++) s.cell.rk[j] = rkpt[j];
+    // columns: type -> l -> m -> atom
+    s.vstride = (long)mxdlqp + 3;
+    s.types.assign(ntype, KbType{});
+    s.tab.assign((size_t)std::max(ntype, 1) * 4 * s.vstride, 0.0);
+    s.atoms.clear();
+    s.xnl.clear();
+    int ind = 0;
+    for (int k = 0; k < ntype; k++) {
+        KbType &ty = s.types[k];
+        if (natom[k] < 0 || natom[k] > mxdatm)
+            return fail(CPWGPU_ERR_ARG, "natom(%d) = %d, mxdatm = %d", k + 1, natom[k], mxdatm);
+        ty.natom = natom[k];
+        ty.nq = nqnl[k];
+        ty.voff = (long)k * 4 * s.vstride;
+        for (int l = 0; l < 4; l++) {
+            ty.base[l] = ind;
+            const int nk = nkb[l + 4 * (1 + 3 * k)];
+            if (nk == 0) continue;
+            ty.lmask |= 1 << l;
+            ind += (2 * l + 1) * natom[k];
+            for (int c = 0; c < (2 * l + 1) * natom[k]; c++) s.xnl.push_back((double)nk);
+            std::memcpy(&s.tab[ty.voff + l * s.vstride], vkb + s.vstride * (l + 4 * (1 + 3L * k)),
+                        sizeof(double) * s.vstride);
+        }
+        if (ty.lmask) {
+            if (nqnl[k] > mxdlqp + 1)
+                return fail(CPWGPU_ERR_ARG, "nqnl(%d) = %d > mxdlqp + 1 = %d", k + 1, nqnl[k], mxdlqp + 1);
+            if (!(delqnl[k] > 0.0)) return fail(CPWGPU_ERR_ARG, "delqnl(%d) = %g", k + 1, delqnl[k]);
+            ty.inv_delq = 1.0 / delqnl[k];
+            for (int kk = 0; kk < natom[k]; kk++) {
+                KbAtom at{};
+                for (int j = 0; j < 3; j++) at.rat[j] = rat[j + 3 * ((long)kk + (long)mxdatm * k)];
+                at.type = k;
+                at.atom = kk;
+                s.atoms.push_back(at);
+            }
+        }
+    }
+    if (ind != nanl) return fail(CPWGPU_ERR_ARG, "proj_nl_kb: inconsistent value of nanl %d should be %d", nanl, ind);
+    if (s.atoms.size() > 65535) return fail(CPWGPU_ERR_SIZE, "%zu atoms with projectors", s.atoms.size());
+    return CPWGPU_OK;
+}
+
+// anl(lda, nanl) complex128 on the device from the g_j of the plane waves
+// (gw_dev[j * mtxd + i]); enqueued on g_stream.
+int kb_build(const KbSetup &s, const double *gw_dev, int mtxd, double2 *anl, long lda) {
+    if (mtxd == 0 || s.atoms.empty()) return CPWGPU_OK;
+    int st;
+    if ((st = upload(g_kb_tab, s.tab.data(), s.tab.size())) ||
+        (st = upload(g_kb_types, s.types.data(), s.types.size())) ||
+        (st = upload(g_kb_atoms, s.atoms.data(), s.atoms.size())))
+        return st;
+    const dim3 grid((unsigned)((mtxd + KB_THREADS - 1) / KB_THREADS), (unsigned)s.atoms.size());
+    ProfScope ps(CPWGPU_K_PROJ_KB, g_stream);
+    hipLaunchKernelGGL(k_proj_kb, grid, dim3(KB_THREADS), 0, g_stream, gw_dev, mtxd, s.cell,
+                       (const KbAtom *)g_kb_atoms.p, (const KbType *)g_kb_types.p, (const double *)g_kb_tab.p,
+                       s.vstride, anl, lda);
+    HIPCHK(hipGetLastError());
     return CPWGPU_OK;
 }
 
@@ -2118,6 +2240,7 @@ int cpwgpu_finalize(void) {
     g_drop.hd.release();
     g_drop.c8.release();
     g_spin_tmp.release();
+    g_kb_tab.release(); g_kb_types.release(); g_kb_atoms.release(); g_kb_gw.release(); g_kb_out.release();
     if (g_blas) {
         if (g_stream) HIPCHK(hipStreamSynchronize(g_stream));
         rocblas_destroy_handle(g_blas);
@@ -2218,6 +2341,61 @@ int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_de
         HIPCHK(hipMemcpy2DAsync(plan->anl64.p, 16 * (size_t)plan->lda, anlga_dev, 16 * (size_t)lda,
                                 16 * (size_t)plan->mtxd, nanl, hipMemcpyDeviceToDevice, g_stream));
     HIPCHK(hipMemcpyAsync(plan->xnl.p, xnlkb_dev, 8 * (size_t)nanl, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_plan_build_projectors_kb(cpwgpu_plan_t plan, const double *rkpt, int nanl, const int *nqnl,
+                                    const double *delqnl, const double *vkb, const int *nkb, int ntype,
+                                    const int *natom, const double *rat, const double *adot, int mxdtyp,
+                                    int mxdatm, int mxdlqp) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!plan || nanl < 0) return fail(CPWGPU_ERR_ARG, "bad argument");
+    KbSetup s;
+    int st = kb_setup(rkpt, nanl, nqnl, delqnl, vkb, nkb, ntype, natom, rat, adot, mxdtyp, mxdatm, mxdlqp, s);
+    if (st || (st = ensure_stream())) return st;
+    plan->nanl = nanl;
+    plan->anl32_valid = false;
+    if (nanl == 0) return CPWGPU_OK;
+    plan->lda = std::max(1, plan->mtxd);
+    if ((st = plan->anl64.reserve(16 * (size_t)plan->lda * nanl)) || (st = upload(plan->xnl, s.xnl.data(), nanl)) ||
+        (st = kb_build(s, (const double *)plan->gw.p, plan->mtxd, (double2 *)plan->anl64.p, plan->lda)))
+        return st;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return CPWGPU_OK;
+}
+
+int cpwgpu_proj_nl_kb_c16(const double *rkpt, const int *mtxd, const int *isort, const int *nanl, const int *ng,
+                          const int *kgv, const int *nqnl, const double *delqnl, const double *vkb, const int *nkb,
+                          const int *ntype, const int *natom, const double *rat, const double *adot, double *anlga,
+                          double *xnlkb, const int *mxdtyp, const int *mxdatm, const int *mxdlqp, const int *mxddim,
+                          const int *mxdanl, const int *mxdgve) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!mtxd || !nanl || !ng || !ntype || !mxdtyp || !mxdatm || !mxdlqp || !mxddim || !mxdanl || !mxdgve)
+        return fail(CPWGPU_ERR_ARG, "null dimension argument");
+    if (*ng > *mxdgve) return fail(CPWGPU_ERR_NG, "ng = %d > mxdgve = %d", *ng, *mxdgve);
+    if (*mtxd < 0 || *mtxd > *mxddim) return fail(CPWGPU_ERR_MTXD, "mtxd = %d, mxddim = %d", *mtxd, *mxddim);
+    KbSetup s;
+    int st = kb_setup(rkpt, *nanl, nqnl, delqnl, vkb, nkb, *ntype, natom, rat, adot, *mxdtyp, *mxdatm, *mxdlqp, s);
+    if (st) return st;
+    if (*nanl > *mxdanl) return fail(CPWGPU_ERR_ARG, "proj_nl_kb: increase mxdanl from %d to %d", *mxdanl, *nanl);
+    const int m = *mtxd, na = *nanl;
+    if (na == 0) return CPWGPU_OK;
+    if (!xnlkb || !anlga) return fail(CPWGPU_ERR_ARG, "null anlga/xnlkb");
+    std::memcpy(xnlkb, s.xnl.data(), sizeof(double) * na);
+    if (m == 0) return CPWGPU_OK;
+    if (!isort || !kgv) return fail(CPWGPU_ERR_ARG, "null isort/kgv");
+    std::vector<double> gwh(3 * (size_t)m);
+    for (int i = 0; i < m; i++) {
+        if (isort[i] < 1 || isort[i] > *ng) return fail(CPWGPU_ERR_ARG, "isort(%d) = %d outside 1..ng", i + 1, isort[i]);
+        for (int j = 0; j < 3; j++) gwh[(size_t)j * m + i] = (double)kgv[3L * (isort[i] - 1) + j];
+    }
+    if ((st = ensure_stream()) || (st = upload(g_kb_gw, gwh.data(), gwh.size())) ||
+        (st = g_kb_out.reserve(16 * (size_t)m * na)) ||
+        (st = kb_build(s, (const double *)g_kb_gw.p, m, (double2 *)g_kb_out.p, m)))
+        return st;
+    HIPCHK(hipMemcpy2DAsync(anlga, 16 * (size_t)*mxddim, g_kb_out.p, 16 * (size_t)m, 16 * (size_t)m, na,
+                            hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
 }

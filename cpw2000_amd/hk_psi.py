@@ -51,13 +51,14 @@ EXPORTS = (
     "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
     "cpwgpu_hk_psi_spin_c16", "cpwgpu_plan_set_potential_spin", "cpwgpu_plan_set_projectors_spin",
     "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8", "cpwgpu_hk_psi_allk_c16",
+    "cpwgpu_proj_nl_kb_c16", "cpwgpu_plan_build_projectors_kb",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
 TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN = 0, 1, 2, 3, 4, 5
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
-KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back")
+KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back", "k_proj_kb")
 
 
 class CpwGpuError(RuntimeError):
@@ -93,6 +94,8 @@ def lib():
         L.cpwgpu_plan_set_potential.argtypes = [vp, dp, C.c_int]
         L.cpwgpu_plan_set_projectors.argtypes = [vp, dp, dp, C.c_int, C.c_int]
         L.cpwgpu_plan_set_projectors_device.argtypes = [vp, dp, C.c_long, dp, C.c_int]
+        L.cpwgpu_plan_build_projectors_kb.argtypes = [vp, dp, C.c_int, vp, dp, dp, vp, C.c_int, vp, dp, dp,
+                                                      C.c_int, C.c_int, C.c_int]
         L.cpwgpu_plan_apply.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_plan_info.argtypes = [vp, ip]
         L.cpwgpu_profile.argtypes = [C.c_int, C.c_uint]
@@ -161,6 +164,56 @@ def hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl, ng, kgv, ekpg, isort, vscr, kmscr
         _iref(mxdbnd), _iref(mxdanl), _iref(mxdgve), _iref(mxdscr))
     _check(st)
     return hpsi
+
+
+def _kb_tables(nqnl, delqnl, vkb, nkb, natom, rat, adot):
+    """The pseudopotential arrays of proj_nl_kb_c16 as Fortran-ordered numpy
+    arrays of the reference's kinds; vkb (mxdlqp + 3, 4, 3, mxdtyp) with its
+    first index shifted by 2, nkb (4, 3, mxdtyp), rat (3, mxdatm, mxdtyp)."""
+    out = []
+    for name, arr, dt, nd in (("nqnl", nqnl, np.int32, 1), ("delqnl", delqnl, np.float64, 1),
+                              ("vkb", vkb, np.float64, 4), ("nkb", nkb, np.int32, 3),
+                              ("natom", natom, np.int32, 1), ("rat", rat, np.float64, 3),
+                              ("adot", adot, np.float64, 2)):
+        a = np.asfortranarray(arr, dt)
+        if a.ndim != nd:
+            raise ValueError(f"{name} must have {nd} dimensions")
+        out.append(a)
+    nq, dq, v, nk, na, r, ad = out
+    if v.shape[1:3] != (4, 3) or nk.shape[:2] != (4, 3) or ad.shape != (3, 3) or r.shape[0] != 3:
+        raise ValueError("vkb (mxdlqp+3, 4, 3, mxdtyp), nkb (4, 3, mxdtyp), rat (3, mxdatm, mxdtyp), adot (3, 3)")
+    mxdtyp = v.shape[3]
+    if nk.shape[2] != mxdtyp or r.shape[2] != mxdtyp or nq.shape[0] < mxdtyp or dq.shape[0] < mxdtyp \
+            or na.shape[0] < mxdtyp:
+        raise ValueError("per-type arrays must cover mxdtyp types")
+    return nq, dq, v, nk, na, r, ad, mxdtyp, r.shape[1], v.shape[0] - 3
+
+
+def proj_nl_kb_c16(rkpt, mtxd, isort, nanl, ng, kgv, nqnl, delqnl, vkb, nkb, ntype, natom, rat, adot,
+                   anlga, xnlkb, mxddim, mxdanl, mxdgve):
+    """KB projectors with the arguments of Src/omp/proj_nl_kb_c16.f90:23-28,
+    computed on the GPU (cpwgpu_proj_nl_kb_c16).  anlga complex128 (mxddim,
+    mxdanl) and xnlkb float64 (mxdanl,) are written in place (rows 0..mtxd-1,
+    columns 0..nanl-1); mxdtyp, mxdatm and mxdlqp come from the shapes of
+    vkb (mxdlqp + 3, 4, 3, mxdtyp) and rat (3, mxdatm, mxdtyp)."""
+    nq, dq, v, nk, na, r, ad, mxdtyp, mxdatm, mxdlqp = _kb_tables(nqnl, delqnl, vkb, nkb, natom, rat, adot)
+    for name, arr, dt in (("isort", isort, np.int32), ("kgv", kgv, np.int32),
+                          ("anlga", anlga, np.complex128), ("xnlkb", xnlkb, np.float64)):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{name} must be a Fortran-contiguous {np.dtype(dt).name} array")
+    if anlga.ndim != 2 or anlga.shape[0] != mxddim or anlga.shape[1] < min(nanl, mxdanl) \
+            or xnlkb.size < min(nanl, mxdanl) or isort.size < mtxd or kgv.shape[0] != 3 \
+            or kgv.shape[1] < min(ng, mxdgve):
+        raise ValueError("array shapes do not match mxddim, mxdanl, nanl, mtxd, ng")
+    rk = np.ascontiguousarray(rkpt, np.float64)
+    if rk.size != 3:
+        raise ValueError("rkpt must have 3 components")
+    st = lib().cpwgpu_proj_nl_kb_c16(
+        _p(rk), _iref(mtxd), _p(isort), _iref(nanl), _iref(ng), _p(kgv), _p(nq), _p(dq), _p(v), _p(nk),
+        _iref(ntype), _p(na), _p(r), _p(ad), _p(anlga), _p(xnlkb), _iref(mxdtyp), _iref(mxdatm),
+        _iref(mxdlqp), _iref(mxddim), _iref(mxdanl), _iref(mxdgve))
+    _check(st)
+    return anlga, xnlkb
 
 
 def hk_psi_allk_c16(nrk, mtxd_allk, nband, psi_allk, hpsi_allk, ng, kgv, ekpg_allk, isort_allk, vscr,
@@ -395,6 +448,18 @@ class Plan:
         x = np.ascontiguousarray(xnlkb, np.float64)
         nanl = a.shape[1] if nanl is None else nanl
         _check(lib().cpwgpu_plan_set_projectors(self._h, _p(a), _p(x), int(nanl), int(a.shape[0])))
+
+    def build_projectors_kb(self, rkpt, nanl, nqnl, delqnl, vkb, nkb, ntype, natom, rat, adot):
+        """The projectors of proj_nl_kb_c16 (Src/omp/proj_nl_kb_c16.f90) built on
+        the device for this plan's plane waves (cpwgpu_plan_build_projectors_kb);
+        the arrays as for proj_nl_kb_c16 above."""
+        nq, dq, v, nk, na, r, ad, mxdtyp, mxdatm, mxdlqp = _kb_tables(nqnl, delqnl, vkb, nkb, natom, rat, adot)
+        rk = np.ascontiguousarray(rkpt, np.float64)
+        if rk.size != 3:
+            raise ValueError("rkpt must have 3 components")
+        _check(lib().cpwgpu_plan_build_projectors_kb(
+            self._h, _p(rk), int(nanl), _p(nq), _p(dq), _p(v), _p(nk), int(ntype), _p(na), _p(r), _p(ad),
+            int(mxdtyp), int(mxdatm), int(mxdlqp)))
 
     def set_projectors_device(self, anlga_t, xnlkb_t):
         """anlga_t: complex128 device tensor (nanl, lda) (each projector contiguous)."""

@@ -320,6 +320,47 @@ def projectors(spec, kgv, isort, kpt, bvec, omega, mxddim=None):
     return anl, np.array(xnl, float)
 
 
+def kb_pseudo(spec, delq=0.002, lsets=None, tables=None, nqnl=None, nkb_sign=None, mxdatm_pad=0):
+    """The pseudopotential inputs of the projector construction
+    Src/omp/proj_nl_kb_c16.f90:23-73 in the reference's layouts (numpy,
+    Fortran order): adot(3,3), natom(ntype), rat(3, mxdatm, ntype),
+    nkb(0:3, -1:1, ntype) as (4, 3, ntype), vkb(-2:mxdlqp, 0:3, -1:1, ntype)
+    as (mxdlqp + 3, 4, 3, ntype), nqnl and delqnl per type, nanl.
+
+    Default: s and p projectors with the Gaussian radial part of
+    ``projectors`` (vkb_l(q) = exp(-q^2 rc^2 / 2); rylm carries q^l),
+    tabulated every ``delq`` up to |k+G| = sqrt(2 Ecut) + 3 bohr^-1.
+    ``lsets[t]`` (the l values of type t), ``tables(t, l, q)`` (the table at
+    |q|, an array), ``nqnl`` and ``nkb_sign[t][l]`` replace the defaults."""
+    ntype = len(spec.types)
+    natom = np.array([frac.shape[0] for _, frac, _ in spec.types], np.int32)
+    mxdatm = int(natom.max()) + mxdatm_pad
+    rat = np.zeros((3, mxdatm, ntype), order="F")
+    for t, (_, frac, _) in enumerate(spec.types):
+        rat[:, :frac.shape[0], t] = frac.T
+    delqnl = np.broadcast_to(np.asarray(delq, float), (ntype,)).copy()
+    if nqnl is None:
+        qmax = math.sqrt(2.0 * spec.ecut) + 3.0
+        nqnl = [int(qmax / dq) + 4 for dq in delqnl]
+    nqnl = np.asarray(nqnl, np.int32)
+    mxdlqp = max(int(nqnl.max()) - 1, 0)    # the last table point nqnl - 1 is index mxdlqp
+    if lsets is None:
+        lsets = [(0, 1)] * ntype
+    if tables is None:
+        rcs = [rc for _, _, rc in spec.types]
+        tables = lambda t, l, q: np.exp(-0.5 * q * q * rcs[t] ** 2)
+    nkb = np.zeros((4, 3, ntype), np.int32, order="F")
+    vkb = np.zeros((mxdlqp + 3, 4, 3, ntype), order="F")
+    j = np.arange(-2, mxdlqp + 1)
+    for t in range(ntype):
+        for l in lsets[t]:
+            nkb[l, 1, t] = 1 if nkb_sign is None else nkb_sign[t][l]
+            vkb[:, l, 1, t] = tables(t, l, np.abs(j) * delqnl[t])
+    nanl = sum((2 * l + 1) * int(natom[t]) for t in range(ntype) for l in lsets[t])
+    return dict(nqnl=nqnl, delqnl=delqnl, vkb=vkb, nkb=nkb, ntype=ntype, natom=natom, rat=rat,
+                adot=np.asfortranarray(spec.avec.T @ spec.avec), nanl=nanl)
+
+
 def wavefunctions(rng, ekpg, nb, mxddim=None):
     """psi = (N(0,1)+iN(0,1))/sqrt2 * exp(-|k+G|^2/8), Loewdin-orthonormalised."""
     mtxd = ekpg.shape[0]

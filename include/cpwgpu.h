@@ -109,6 +109,24 @@ int cpwgpu_hk_psi_c8(const int *mtxd, const int *neig, const float *psi, float *
                      const int *kmscr, const float *anlga, const float *xnlkb, const int *nanl, const int *mxddim,
                      const int *mxdbnd, const int *mxdanl, const int *mxdgve, const int *mxdscr);
 
+/* Drop-in for proj_nl_kb_c16 (Src/omp/proj_nl_kb_c16.f90:23-28, the
+ * reference's argument list, all by reference, host arrays):
+ *   rkpt(3), mtxd, isort(mxddim), nanl, ng, kgv(3,mxdgve), nqnl(mxdtyp),
+ *   delqnl(mxdtyp), vkb(-2:mxdlqp,0:3,-1:1,mxdtyp), nkb(0:3,-1:1,mxdtyp),
+ *   ntype, natom(mxdtyp), rat(3,mxdatm,mxdtyp), adot(3,3) in;
+ *   anlga(mxddim,mxdanl) complex(8), xnlkb(mxdanl) out (rows 1..mtxd and
+ *   columns 1..nanl written, the rest untouched);
+ *   mxdtyp, mxdatm, mxdlqp, mxddim, mxdanl, mxdgve dimensions.
+ * The projectors are computed on the GPU and copied back.  ng > mxdgve is
+ * CPWGPU_ERR_NG (:117), nanl inconsistent with nkb/natom or > mxdanl
+ * CPWGPU_ERR_ARG (:143-157), with the conditions of
+ * cpwgpu_plan_build_projectors_kb below. */
+int cpwgpu_proj_nl_kb_c16(const double *rkpt, const int *mtxd, const int *isort, const int *nanl, const int *ng,
+                          const int *kgv, const int *nqnl, const double *delqnl, const double *vkb, const int *nkb,
+                          const int *ntype, const int *natom, const double *rat, const double *adot, double *anlga,
+                          double *xnlkb, const int *mxdtyp, const int *mxdatm, const int *mxdlqp, const int *mxddim,
+                          const int *mxdanl, const int *mxdgve);
+
 /* ---- device-resident plan ------------------------------------------------ */
 
 /* Builds the index maps of one k-point from host arrays (the hk_psi_c16
@@ -125,6 +143,20 @@ int cpwgpu_plan_set_projectors(cpwgpu_plan_t plan, const double *anlga, const do
 /* Same from device memory (anlga complex128 with leading dimension lda). */
 int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_dev, long lda,
                                       const double *xnlkb_dev, int nanl);
+/* Builds the projectors of the plan's k-point on the device, as
+ * proj_nl_kb_c16 (Src/omp/proj_nl_kb_c16.f90:23-275) computes anlga and xnlkb
+ * for the plan's plane waves: rkpt(3), nqnl(mxdtyp), delqnl(mxdtyp),
+ * vkb(-2:mxdlqp, 0:3, -1:1, mxdtyp), nkb(0:3, -1:1, mxdtyp), natom(mxdtyp),
+ * rat(3, mxdatm, mxdtyp), adot(3, 3) in the reference's layouts (host memory).
+ * Nothing of size mtxd * nanl crosses PCIe.  A nanl that does not match nkb
+ * and natom, or a cell with a non-positive metric, is CPWGPU_ERR_ARG (the
+ * reference's stops, proj_nl_kb_c16.f90:143-157 and Src/crystal/adot_to_avec.f90);
+ * so is nqnl(type) > mxdlqp + 1 or delqnl(type) <= 0 for a type with
+ * projectors (a table read beyond vkb's bounds in the reference). */
+int cpwgpu_plan_build_projectors_kb(cpwgpu_plan_t plan, const double *rkpt, int nanl, const int *nqnl,
+                                    const double *delqnl, const double *vkb, const int *nkb, int ntype,
+                                    const int *natom, const double *rat, const double *adot, int mxdtyp,
+                                    int mxdatm, int mxdlqp);
 /* H|psi> on device arrays: psi_dev(ldp, neig), hpsi_dev(ldh, neig) complex128,
  * rows 0..mtxd-1 of hpsi written.  stream is a hipStream_t (NULL = the null
  * stream).  Returns when the work is enqueued. */
@@ -343,7 +375,8 @@ int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long
 #define CPWGPU_K_PROJ 3          /* <beta|psi> split-K GEMM                        */
 #define CPWGPU_K_PROJ_REDUCE 4   /* split-K sum and D scaling                      */
 #define CPWGPU_K_BACK 5          /* hpsi += beta D<beta|psi> GEMM                  */
-#define CPWGPU_NKERNEL 6
+#define CPWGPU_K_PROJ_KB 6       /* KB projector construction                      */
+#define CPWGPU_NKERNEL 7
 /* When on, later applies record a HIP event pair around each launch of the
  * kernels selected by mask (bit k = kernel k) on the launch stream. */
 int cpwgpu_profile(int on, unsigned mask);

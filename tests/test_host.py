@@ -82,6 +82,44 @@ def test_fortran_shim_exports_hk_psi_c16_and_ditsp_c16():
     assert re.search(r"\bT ditsp_c16_\b", out)
     assert re.search(r"\bT hk_psi_spin_c16_\b", out)
     assert re.search(r"\bT hk_psi_allk_c16_\b", out)
+    assert re.search(r"\bT proj_nl_kb_c16_\b", out)
+
+
+def test_proj_nl_kb_checks_before_any_device_work():
+    """cpwgpu_proj_nl_kb_c16 and cpwgpu_plan_build_projectors_kb stop where
+    the reference stops (Src/omp/proj_nl_kb_c16.f90:117, :143-157; the cell of
+    Src/crystal/adot_to_avec.f90), and reject tables that a read at
+    nint(|q|/delqnl) + 1 would overrun, before they touch the device."""
+    from cpw2000_amd import hk_psi
+    from tests.kb_case import mixed_case
+
+    _, _, d = mixed_case()
+    m, nanl = d["mtxd"], d["nanl"]
+
+    def call(**over):
+        a = dict(d, anlga=np.zeros((m, nanl), np.complex128, order="F"), xnlkb=np.zeros(nanl),
+                 mxddim=m, mxdanl=nanl, mxdgve=d["kgv"].shape[1])
+        a.update(over)
+        hk_psi.proj_nl_kb_c16(**a)
+
+    bad_adot = np.asfortranarray(np.diag([1.0, 1.0, -1.0]))
+    short = np.asfortranarray(d["nqnl"] + np.array([2, 0], np.int32))   # 701 + 2 > mxdlqp + 1
+    zero_dq = np.array([0.01, 0.0])
+    for over, code in ((dict(nanl=nanl - 1, mxdanl=nanl), hk_psi.ERR_ARG),
+                       (dict(mxdanl=nanl - 1), hk_psi.ERR_ARG),
+                       (dict(ng=d["kgv"].shape[1] + 1), hk_psi.ERR_NG),
+                       (dict(mxddim=m - 1, anlga=np.zeros((m - 1, nanl), np.complex128, order="F")),
+                        hk_psi.ERR_MTXD),
+                       (dict(adot=bad_adot), hk_psi.ERR_ARG),
+                       (dict(nqnl=short), hk_psi.ERR_ARG),
+                       (dict(delqnl=zero_dq), hk_psi.ERR_ARG),
+                       (dict(ntype=3), hk_psi.ERR_ARG)):
+        with pytest.raises(hk_psi.CpwGpuError) as e:
+            call(**over)
+        assert e.value.code == code, over
+    with pytest.raises(hk_psi.CpwGpuError, match="inconsistent value of nanl"):
+        call(nanl=nanl + 1, anlga=np.zeros((m, nanl + 1), np.complex128, order="F"), xnlkb=np.zeros(nanl + 1),
+             mxdanl=nanl + 1)
 
 
 def test_allk_argument_checks_before_any_device_work():

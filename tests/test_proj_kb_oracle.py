@@ -1,0 +1,129 @@
+"""Pins the projector oracle (oracle/proj_kb.py, the restatement of
+Src/omp/proj_nl_kb_c16.f90) without the reference:
+
+* the cell: avec upper triangular with a positive diagonal (the canonical
+  orientation of Src/crystal/adot_to_avec.f90, which fixes it uniquely),
+  avec^T avec = adot, avec^T bvec = 2 pi I, vcell = |det avec|;
+* rylm (Src/numeric/ylm_rc.f90:82-107) = |q|^l sqrt(4 pi / (2l+1)) times the
+  real spherical harmonics built from scipy's complex ones, every l <= 3, m;
+* the interpolation reproduces any quadratic exactly (the three-point
+  Lagrange form) and is zero from nqnl - 1 on;
+* whole projectors against the independently written Gaussian generator of
+  cpw2000_amd/synth.py on cubic cells (one and two species), where the
+  canonical frame is the generator's frame: equal to the table's
+  interpolation error (< 1e-8 relative);
+* on the fcc cell, whose canonical frame is rotated against the generator's,
+  the s columns agree and sum_m |column(l=1, m)|^2 agrees (rotation invariant);
+* column order type -> l -> m -> atom and xnlkb = nkb, and the reference's
+  stops (inconsistent nanl, nanl > mxdanl)."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from cpw2000_amd import synth
+from oracle import proj_kb as orc_kb
+from tests.kb_case import gauss_case, mixed_case, oracle_call
+
+
+def test_cell_is_canonical():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        a0 = rng.standard_normal((3, 3)) + 4 * np.eye(3)
+        adot = a0.T @ a0
+        avec, bvec = orc_kb.adot_to_avec(adot)
+        assert np.allclose(np.tril(avec, -1), 0) and (np.diag(avec) > 0).all()
+        assert np.allclose(avec.T @ avec, adot, atol=1e-12)
+        assert np.allclose(avec.T @ bvec, 2 * np.pi * np.eye(3), atol=1e-12)
+        assert abs(orc_kb.vcell_of(adot) - abs(np.linalg.det(a0))) < 1e-10
+
+
+def test_rylm_is_scaled_real_harmonic():
+    rng = np.random.default_rng(1)
+    q = rng.standard_normal((200, 3))
+    r = np.linalg.norm(q, axis=1)
+    th, ph = np.arccos(q[:, 2] / r), np.arctan2(q[:, 1], q[:, 0])
+    ry = orc_kb.rylm(q)
+    for l in range(4):
+        for m in range(-l, l + 1):
+            y = sp.sph_harm_y(l, abs(m), th, ph)
+            real = y.real if m == 0 else math.sqrt(2) * (-1) ** m * (y.real if m > 0 else y.imag)
+            want = r ** l * math.sqrt(4 * math.pi / (2 * l + 1)) * real
+            assert np.abs(ry[l][m + l] - want).max() < 1e-12 * max(1.0, np.abs(want).max()), (l, m)
+        # addition theorem: sum_m rylm^2 = |q|^(2l)
+        assert np.allclose(sum(v * v for v in ry[l]), r ** (2 * l), rtol=1e-12)
+
+
+def test_interpolation_exact_on_quadratics():
+    delq, nq = 0.05, 40
+    j = np.arange(-2, nq + 1)
+    line = 0.3 - 1.7 * (j * delq) + 2.2 * (j * delq) ** 2
+    qi = np.linspace(0, (nq - 1.5) * delq - 1e-9, 500)
+    v = orc_kb.interp_vkb(line, nq, delq, qi)
+    want = 0.3 - 1.7 * qi + 2.2 * qi ** 2
+    assert np.abs(v - want).max() < 1e-12
+    # nint(|q|/delq) >= nq - 1 -> zero
+    far = np.array([(nq - 1.5) * delq + 1e-9, nq * delq, 10.0])
+    assert (orc_kb.interp_vkb(line, nq, delq, far) == 0).all()
+
+
+@pytest.mark.parametrize("cfg,kpt", [(1, (0.0, 0.0, 0.0)), (4, (0.25, -0.25, 0.125))])
+def test_matches_gaussian_generator_on_cubic_cells(cfg, kpt):
+    spec, ks, d = gauss_case(cfg, kpt, ecut=4.0)
+    anl, xnl = oracle_call(orc_kb, d)
+    omega = abs(np.linalg.det(spec.avec))
+    ref, _ = synth.projectors(spec, ks["kgv"], ks["isort"], ks["kpt"], ks["bvec"], omega)
+    assert anl.shape == ref.shape
+    err = np.abs(anl - ref).max() / np.abs(ref).max()
+    assert err < 1e-8, err
+    assert (xnl == 1.0).all()
+
+
+def test_fcc_frame_rotation_invariants():
+    spec, ks, d = gauss_case(0, (0.1, 0.2, 0.3))
+    anl, _ = oracle_call(orc_kb, d)
+    omega = abs(np.linalg.det(spec.avec))
+    ref, _ = synth.projectors(spec, ks["kgv"], ks["isort"], ks["kpt"], ks["bvec"], omega)
+    nat = 2
+    scale = np.abs(ref).max()
+    # s columns do not depend on the frame
+    assert np.abs(anl[:, :nat] - ref[:, :nat]).max() < 1e-8 * scale
+    # p columns: sum over m of |.|^2 per atom is invariant
+    for a in range(nat):
+        cols = [nat + m * nat + a for m in range(3)]
+        s1 = (np.abs(anl[:, cols]) ** 2).sum(axis=1)
+        s2 = (np.abs(ref[:, cols]) ** 2).sum(axis=1)
+        assert np.abs(s1 - s2).max() < 1e-8 * scale ** 2
+    # ... while the columns themselves differ (the frames are not the same)
+    assert np.abs(anl[:, nat:] - ref[:, nat:]).max() > 1e-3 * scale
+
+
+def test_column_order_and_signs():
+    _, ks, d = mixed_case()
+    anl, xnl = oracle_call(orc_kb, d, mxddim=d["mtxd"] + 3, mxdanl=d["nanl"] + 2)
+    cols = orc_kb.projector_columns(d["ntype"], d["natom"], d["nkb"])
+    # type A: l = 0, 1, 3 (1 atom each), type B: l = 0, 2, 3
+    assert [(k, l, m) for k, _, l, m in cols] == (
+        [(0, 0, 0)] + [(0, 1, m) for m in (-1, 0, 1)] + [(0, 3, m) for m in range(-3, 4)] +
+        [(1, 0, 0)] + [(1, 2, m) for m in range(-2, 3)] + [(1, 3, m) for m in range(-3, 4)])
+    assert list(xnl[:d["nanl"]]) == [1] + [-1] * 3 + [1] * 7 + [1] + [1] * 5 + [-1] * 7
+    assert (anl[d["mtxd"]:] == 0).all() and (anl[:, d["nanl"]:] == 0).all()
+    # species B's table ends inside the sphere: its columns vanish beyond
+    g = d["kgv"][:, d["isort"] - 1].T + d["rkpt"]
+    _, bvec = orc_kb.adot_to_avec(d["adot"])
+    qi = np.linalg.norm(g @ bvec.T, axis=1)
+    beyond = np.floor(qi / d["delqnl"][1] + 0.5) >= d["nqnl"][1] - 1
+    assert beyond.any() and (~beyond).any()
+    assert (anl[:d["mtxd"]][beyond, 11:d["nanl"]] == 0).all()
+    assert (np.abs(anl[:d["mtxd"]][~beyond, 11]) > 0).all()
+
+
+def test_reference_stops():
+    _, _, d = mixed_case()
+    with pytest.raises(orc_kb.ProjError, match="inconsistent value of nanl"):
+        d2 = dict(d, nanl=d["nanl"] - 1)
+        oracle_call(orc_kb, d2)
+    with pytest.raises(orc_kb.ProjError, match="increase mxdanl"):
+        oracle_call(orc_kb, d, mxdanl=d["nanl"] - 1)
