@@ -6,7 +6,10 @@ FFT backend and netlib zgemm) is compiled from the cpw2000 sources by
 seeded synthetic inputs of cpw2000_amd.synth.  Each fixture holds the
 generator arguments, digests of the inputs (to detect generator drift) and
 the reference output.  Also stored: the reference's pruned 3-D FFT
-(cfft_wf_c16) on a random sphere, both directions.
+(cfft_wf_c16) on a random sphere, both directions, and the reference's
+projector construction (proj_nl_kb_c16, with ylm_rc, adot_to_avec and
+adot_to_bdot from the same build) on the two-species fcc case of
+tests/kb_case.py.
 
 Run in the build container only (it needs the reference sources):
     python tests/golden/make_golden.py
@@ -86,6 +89,40 @@ def make_fft_fixture():
                         chd_out_fwd=out[1])
 
 
+def make_projkb_fixture():
+    """Reference proj_nl_kb_c16 on tests/kb_case.mixed_case (fcc metric, two
+    species, l = 0..3, signs of nkb, random tables, one ending inside the
+    sphere, k off the origin) with padded mxddim / mxdanl / mxdatm."""
+    from tests.kb_case import mixed_case
+
+    lib = O.ref_lib()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    i = lambda v: C.byref(C.c_int(int(v)))
+    meta = []
+    out = {}
+    for tag, kw in (("a", dict()), ("b", dict(kpt=(0.0, 0.5, 0.0), seed=9))):
+        _, _, d = mixed_case(**kw)
+        m, nanl = d["mtxd"], d["nanl"]
+        mxddim, mxdanl = m + 3, nanl + 2
+        anl = np.zeros((mxddim, mxdanl), complex, order="F")
+        xnl = np.zeros(mxdanl)
+        vkb = np.asfortranarray(d["vkb"])
+        lib.proj_nl_kb_c16_(p(np.ascontiguousarray(d["rkpt"])), i(m), p(d["isort"]), i(nanl), i(d["ng"]),
+                            p(np.asfortranarray(d["kgv"], np.int32)), p(d["nqnl"]), p(d["delqnl"]), p(vkb),
+                            p(np.asfortranarray(d["nkb"], np.int32)), i(d["ntype"]), p(d["natom"]),
+                            p(np.asfortranarray(d["rat"])), p(np.asfortranarray(d["adot"])), p(anl), p(xnl),
+                            i(vkb.shape[3]), i(d["rat"].shape[1]), i(vkb.shape[0] - 3), i(mxddim), i(mxdanl),
+                            i(d["kgv"].shape[1]))
+        w = np.cos(0.37 * np.arange(vkb.size))
+        meta.append(dict(tag=tag, kwargs=kw, mtxd=int(m), nanl=int(nanl),
+                         digest_vkb=float(np.dot(w, vkb.reshape(-1, order="F"))),
+                         digest_isort=int(np.dot(np.arange(m) % 97, d["isort"].astype(np.int64)))))
+        out[f"anlga_{tag}"] = anl[:m, :nanl]
+        out[f"xnlkb_{tag}"] = xnl[:nanl]
+        print("proj_nl_kb_c16", tag, m, nanl, "|anlga| =", np.linalg.norm(anl))
+    np.savez_compressed(os.path.join(HERE, "projkb_fcc2.npz"), meta=json.dumps(meta), **out)
+
+
 def main():
     if not O.build_ref():
         raise SystemExit("reference sources not found: cannot write golden vectors")
@@ -99,6 +136,7 @@ def main():
                             hpsi=hpsi[: d["mtxd"]])
         print(name, d["mtxd"], nb, "|hpsi| =", np.linalg.norm(hpsi))
     make_fft_fixture()
+    make_projkb_fixture()
 
 
 if __name__ == "__main__":

@@ -50,3 +50,22 @@ def mixed_case(kpt=(0.13, -0.21, 0.37), seed=5):
                          nkb_sign=[{0: 1, 1: -1, 3: 1}, {0: 1, 2: 1, 3: -1}], mxdatm_pad=2)
     assert pp["vkb"].shape[0] == mxdlqp + 3
     return spec, ks, _case(spec, ks, pp)
+
+
+def golden_cases(path):
+    """(inputs, reference anlga, reference xnlkb) of each case stored by
+    tests/golden/make_golden.py (make_projkb_fixture), after checking that
+    the generator still produces the stored inputs."""
+    import json
+
+    g = np.load(path)
+    out = []
+    for mt in json.loads(str(g["meta"])):
+        _, _, d = mixed_case(**mt["kwargs"])
+        vkb = np.asfortranarray(d["vkb"])
+        w = np.cos(0.37 * np.arange(vkb.size))
+        assert abs(float(np.dot(w, vkb.reshape(-1, order="F"))) - mt["digest_vkb"]) < 1e-9
+        assert int(np.dot(np.arange(d["mtxd"]) % 97, d["isort"].astype(np.int64))) == mt["digest_isort"]
+        assert (d["mtxd"], d["nanl"]) == (mt["mtxd"], mt["nanl"])
+        out.append((d, g["anlga_" + mt["tag"]], g["xnlkb_" + mt["tag"]]))
+    return out

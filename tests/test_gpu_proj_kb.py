@@ -6,6 +6,8 @@ max |anlga|:
   to 3, signs of nkb, random tables (one cut short inside the sphere), a
   k-point off the origin, padded mxddim / mxdanl / mxdatm: rows beyond mtxd
   and columns beyond nanl keep what the caller had there;
+* the same against the reference's own proj_nl_kb_c16 output (golden
+  vectors of tests/golden/make_golden.py, two fcc cases);
 * the Fortran shim proj_nl_kb_c16_ (libcpwgpu_fortran.so), by reference;
 * the 64-atom Si cell (23,847 plane waves, 256 projectors) built into a plan:
   the projectors equal the oracle's, and H|psi> with them equals H|psi>
@@ -17,7 +19,8 @@ import numpy as np
 import pytest
 
 from oracle import proj_kb as orc_kb
-from tests.kb_case import gauss_case, mixed_case, oracle_call
+from tests.conftest import ROOT
+from tests.kb_case import gauss_case, golden_cases, mixed_case, oracle_call
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +55,18 @@ def test_dropin_matches_oracle(hk):
     assert _err(anl[:m, :nanl], ref[:m, :nanl]) < TOL
     assert (anl[m:] == sentinel).all() and (anl[:, nanl:] == sentinel).all()
     assert (xnl[:nanl] == xref[:nanl]).all() and (xnl[nanl:] == -9.0).all()
+
+
+def test_dropin_matches_reference_golden(hk):
+    for d, ref, xref in golden_cases(f"{ROOT}/tests/golden/projkb_fcc2.npz"):
+        m, nanl = d["mtxd"], d["nanl"]
+        anl = np.zeros((m, nanl), np.complex128, order="F")
+        xnl = np.zeros(nanl)
+        hk.proj_nl_kb_c16(d["rkpt"], m, d["isort"], nanl, d["ng"], d["kgv"], d["nqnl"], d["delqnl"],
+                          d["vkb"], d["nkb"], d["ntype"], d["natom"], d["rat"], d["adot"], anl, xnl, m, nanl,
+                          d["kgv"].shape[1])
+        assert _err(anl, ref) < TOL
+        assert (xnl == xref).all()
 
 
 def test_fortran_shim_matches_oracle(hk):

@@ -1,5 +1,9 @@
 """Pins the projector oracle (oracle/proj_kb.py, the restatement of
-Src/omp/proj_nl_kb_c16.f90) without the reference:
+Src/omp/proj_nl_kb_c16.f90) against the reference's own proj_nl_kb_c16
+(golden vectors of tests/golden/make_golden.py: the flang build of the
+reference sources, two fcc cases with two species, l = 0..3 and random
+tables, <= 1e-12; the residue, ~1.5e-13, is rounding of q amplified by the
+slopes of the random tables), and without the reference:
 
 * the cell: avec upper triangular with a positive diagonal (the canonical
   orientation of Src/crystal/adot_to_avec.f90, which fixes it uniquely),
@@ -25,7 +29,17 @@ import scipy.special as sp
 
 from cpw2000_amd import synth
 from oracle import proj_kb as orc_kb
-from tests.kb_case import gauss_case, mixed_case, oracle_call
+from tests.kb_case import gauss_case, golden_cases, mixed_case, oracle_call
+from tests.conftest import ROOT
+
+
+def test_oracle_matches_reference_golden():
+    cases = golden_cases(f"{ROOT}/tests/golden/projkb_fcc2.npz")
+    assert len(cases) == 2
+    for d, ref, xref in cases:
+        anl, xnl = oracle_call(orc_kb, d)
+        assert np.abs(anl - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert (xnl == xref).all()
 
 
 def test_cell_is_canonical():
