@@ -99,19 +99,36 @@ def test_plan_build_si64_and_apply(hk):
     ref, xref = oracle_call(orc_kb, d)
     cfg = synth.make_config(1, nbands=8)
     assert cfg["mtxd"] == m and (cfg["isort"] == d["isort"]).all()
-    plan = hk.Plan.from_inputs(cfg, projectors=False)
+    # the synthetic generator's projectors first, with an FP32-mixed apply, so
+    # that the plan holds an FP32 copy which the device build must replace
+    plan = hk.Plan.from_inputs(cfg, projectors=True)
+    psi = torch.from_numpy(np.ascontiguousarray(cfg["psi"][:m].T)).to("cuda")
+    h32_old = torch.zeros_like(psi)
+    plan.apply(psi, h32_old, hk.PREC_FP32)
+    torch.cuda.synchronize()
     plan.build_projectors_kb(d["rkpt"], nanl, d["nqnl"], d["delqnl"], d["vkb"], d["nkb"], d["ntype"],
                              d["natom"], d["rat"], d["adot"])
-    psi = torch.from_numpy(np.ascontiguousarray(cfg["psi"][:m].T)).to("cuda")
     h_dev = torch.zeros_like(psi)
     plan.apply(psi, h_dev, hk.PREC_FP64)
+    h32_dev = torch.zeros_like(psi)
+    plan.apply(psi, h32_dev, hk.PREC_FP32)
     torch.cuda.synchronize()
     plan.set_projectors(ref, xref)
     h_host = torch.zeros_like(psi)
     plan.apply(psi, h_host, hk.PREC_FP64)
+    h32_host = torch.zeros_like(psi)
+    plan.apply(psi, h32_host, hk.PREC_FP32)
     torch.cuda.synchronize()
     a, b = h_dev.cpu().numpy(), h_host.cpu().numpy()
     assert np.linalg.norm(a - b) <= TOL * np.linalg.norm(b)
+    # FP32-mixed with the device-built projectors: the FP32 copy follows the build
+    a32, b32 = h32_dev.cpu().numpy(), h32_host.cpu().numpy()
+    assert np.linalg.norm(a32 - b32) <= 2e-6 * np.linalg.norm(b32)
+    assert np.linalg.norm(a32 - b) <= 2e-6 * np.linalg.norm(b)
+    # H with the generator's projectors (xnlkb 1.5 / 0.8 instead of nkb = 1) differs by
+    # about 5e-5 of |H psi| (the kinetic term dominates): a stale FP32 copy would fail above
+    old = h32_old.cpu().numpy()
+    assert np.linalg.norm(old - b) > 1e-5 * np.linalg.norm(b)
     # the same projectors through the drop-in, read back
     anl = np.zeros((m, nanl), np.complex128, order="F")
     xnl = np.zeros(nanl)
