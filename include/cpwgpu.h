@@ -152,7 +152,9 @@ int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_de
  * and natom, or a cell with a non-positive metric, is CPWGPU_ERR_ARG (the
  * reference's stops, proj_nl_kb_c16.f90:143-157 and Src/crystal/adot_to_avec.f90);
  * so is nqnl(type) > mxdlqp + 1 or delqnl(type) <= 0 for a type with
- * projectors (a table read beyond vkb's bounds in the reference). */
+ * projectors (a table read beyond vkb's bounds in the reference).  Returns
+ * when the projectors are built; as with cpwgpu_plan_set_projectors, applies
+ * of this plan still running on other streams must have finished first. */
 int cpwgpu_plan_build_projectors_kb(cpwgpu_plan_t plan, const double *rkpt, int nanl, const int *nqnl,
                                     const double *delqnl, const double *vkb, const int *nkb, int ntype,
                                     const int *natom, const double *rat, const double *adot, int mxdtyp,
