@@ -366,15 +366,17 @@ def bench_forces(torch, hk, synth, np, steps):
             "max_force_ha_bohr": round(float(np.abs(f).max()), 6)}
 
 
-def bench_projectors(torch, hk, synth, np, d, steps):
+def bench_projectors(torch, hk, synth, np, d, steps, cfg=1,
+                     workload="Si64 L=20.52, Ecut 15 Ha: 256 s+p KB projectors on 23,847 plane waves"):
     """KB projectors built on the device (cpwgpu_plan_build_projectors_kb, the
-    construction of Src/omp/proj_nl_kb_c16.f90) for the 64-atom Si cell: 256
-    s and p projectors on 23,847 plane waves from interpolation tables of
-    4,242 points per (species, l).  Kernel time from HIP events around each
-    launch; per-call time includes the table upload and the synchronisation.
-    Algorithmic HBM bytes per launch: the projectors written once (16 B per
-    element) plus g of each plane wave read once (24 B)."""
-    spec = synth.config_spec(1)
+    construction of Src/omp/proj_nl_kb_c16.f90) for the cell of ``d``
+    (default the 64-atom Si cell: 256 s and p projectors on 23,847 plane
+    waves) from interpolation tables of 4,242 points per (species, l).  Kernel
+    time from HIP events around each launch; per-call time includes the table
+    upload and the synchronisation.  Algorithmic HBM bytes per launch: the
+    projectors written once (16 B per element) plus g of each plane wave read
+    once (24 B)."""
+    spec = synth.config_spec(cfg)
     pp = synth.kb_pseudo(spec)
     plan = hk.Plan.from_inputs(d, projectors=False)
     args = (np.zeros(3), pp["nanl"], pp["nqnl"], pp["delqnl"], pp["vkb"], pp["nkb"], pp["ntype"], pp["natom"],
@@ -394,8 +396,7 @@ def bench_projectors(torch, hk, synth, np, d, steps):
     us = ms / max(cnt, 1) * 1e3
     nbytes = 16 * mtxd * nanl + 24 * mtxd
     gbs = nbytes / (us * 1e-6) / 1e9
-    return {"workload": "Si64 L=20.52, Ecut 15 Ha: 256 s+p KB projectors on 23,847 plane waves",
-            "nanl": nanl, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
+    return {"workload": workload, "nanl": nanl, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
             "us_per_kernel": round(us, 2), "ms_per_call": round(ms_call, 3),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(gbs / PEAK_HBM_GBS, 3), "bytes_per_launch": nbytes}}
@@ -429,6 +430,9 @@ def bench_cfg2(torch, hk, synth, np, steps):
         out[prec] = {"ms_per_step": round(ms, 4), "bands_per_s": round(nb / (ms * 1e-3), 1),
                      "tuned": c}
     out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
+    out["projectors_kb"] = bench_projectors(
+        torch, hk, synth, np, d, 10, cfg=2,
+        workload="Si216 L=30.78, Ecut 15 Ha: 864 s+p KB projectors on 80,725 plane waves")
     out["hpsi_rel_err_fp32"] = float(torch.linalg.norm(h["fp32"] - h["fp64"]) /
                                      torch.linalg.norm(h["fp64"]))
     s_mat = (x.conj() @ x.T).cpu().numpy()
