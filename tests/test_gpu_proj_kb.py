@@ -11,7 +11,11 @@ max |anlga|:
 * the Fortran shim proj_nl_kb_c16_ (libcpwgpu_fortran.so), by reference;
 * the 64-atom Si cell (23,847 plane waves, 256 projectors) built into a plan:
   the projectors equal the oracle's, and H|psi> with them equals H|psi>
-  with the oracle's projectors uploaded from the host."""
+  with the oracle's projectors uploaded from the host;
+* the non-local forces of the two-species Si/Ge cell with projectors built
+  on the device equal central differences (h = 1e-4 bohr) of E_NL with the
+  projectors rebuilt on the device at the displaced positions, to 1e-5 of
+  the force (the three largest components)."""
 
 import ctypes as C
 
@@ -135,4 +139,47 @@ def test_plan_build_si64_and_apply(hk):
     hk.proj_nl_kb_c16(d["rkpt"], m, d["isort"], nanl, d["ng"], d["kgv"], d["nqnl"], d["delqnl"], d["vkb"],
                       d["nkb"], d["ntype"], d["natom"], d["rat"], d["adot"], anl, xnl, m, nanl, d["kgv"].shape[1])
     assert _err(anl, ref) < TOL
+    plan.close()
+
+
+def test_nonlocal_forces_with_device_projectors_match_finite_differences(hk):
+    import torch
+
+    from cpw2000_amd import synth
+
+    spec = synth.config_spec(4)
+    d = synth.make_config(4, nbands=8)
+    pp = synth.kb_pseudo(spec)
+    assert pp["nanl"] == d["nanl"]
+    L = float(d["avec"][0, 0])
+    assert np.allclose(d["avec"], L * np.eye(3))
+    plan = hk.Plan.from_inputs(d, projectors=False)
+    m = d["mtxd"]
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:m].T)).to("cuda")
+    rk = np.asarray(d["kpt"], float)
+
+    def build(rat):
+        plan.build_projectors_kb(rk, pp["nanl"], pp["nqnl"], pp["delqnl"], pp["vkb"], pp["nkb"], pp["ntype"],
+                                 pp["natom"], rat, pp["adot"])
+
+    def e_nl(rat):
+        build(rat)
+        _, e = plan.forces(psi, d["rat"], d["avec"], d["proj_atom"])
+        return e
+
+    build(pp["rat"])
+    f, _ = plan.forces(psi, d["rat"], d["avec"], d["proj_atom"])
+    assert np.abs(f).max() > 1e-7, np.abs(f).max()
+    h = 1e-4
+    # the three largest components (species t = atom // 32, atom kk within it, direction j)
+    for flat in np.argsort(np.abs(f).ravel())[-3:]:
+        ia, j = divmod(int(flat), 3)
+        t, kk = divmod(ia, 32)
+        rp = pp["rat"].copy(order="F")
+        rm = pp["rat"].copy(order="F")
+        rp[j, kk, t] += h / L
+        rm[j, kk, t] -= h / L
+        fd = -(e_nl(rp) - e_nl(rm)) / (2 * h)
+        # truncation (h / rc)^2 ~ 3e-9 and rounding ~ 1e-8 of |F|: a 1e-5 relative bound
+        assert abs(f[ia, j] - fd) <= 1e-5 * abs(f[ia, j]) + 1e-12, (ia, j, f[ia, j], fd)
     plan.close()
