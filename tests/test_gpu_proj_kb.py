@@ -183,3 +183,36 @@ def test_nonlocal_forces_with_device_projectors_match_finite_differences(hk):
         # truncation (h / rc)^2 ~ 3e-9 and rounding ~ 1e-8 of |F|: a 1e-5 relative bound
         assert abs(f[ia, j] - fd) <= 1e-5 * abs(f[ia, j]) + 1e-12, (ia, j, f[ia, j], fd)
     plan.close()
+
+
+def test_plan_build_rejects_bad_input_and_keeps_projectors(hk):
+    """A rejected build (inconsistent nanl, a table shorter than nqnl, a
+    metric that is not positive definite) returns CPWGPU_ERR_ARG before any
+    device work: H|psi> with the plan's previous projectors is unchanged."""
+    import torch
+
+    from cpw2000_amd import synth
+
+    d = synth.make_config(0, nbands=4)
+    spec = synth.config_spec(0)
+    pp = synth.kb_pseudo(spec)
+    plan = hk.Plan.from_inputs(d, projectors=False)
+    rk = np.asarray(d["kpt"], float)
+    args = dict(rkpt=rk, nanl=pp["nanl"], nqnl=pp["nqnl"], delqnl=pp["delqnl"], vkb=pp["vkb"], nkb=pp["nkb"],
+                ntype=pp["ntype"], natom=pp["natom"], rat=pp["rat"], adot=pp["adot"])
+    plan.build_projectors_kb(**args)
+    m = d["mtxd"]
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:m].T)).to("cuda")
+    h0 = torch.zeros_like(psi)
+    plan.apply(psi, h0, hk.PREC_FP64)
+    torch.cuda.synchronize()
+    for over in (dict(nanl=pp["nanl"] + 1), dict(nqnl=pp["nqnl"] + 5),
+                 dict(adot=np.asfortranarray(np.diag([1.0, -1.0, 1.0])))):
+        with pytest.raises(hk.CpwGpuError) as e:
+            plan.build_projectors_kb(**dict(args, **over))
+        assert e.value.code == hk.ERR_ARG, over
+    h1 = torch.zeros_like(psi)
+    plan.apply(psi, h1, hk.PREC_FP64)
+    torch.cuda.synchronize()
+    assert torch.equal(h0, h1)
+    plan.close()
