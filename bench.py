@@ -396,10 +396,18 @@ def bench_projectors(torch, hk, synth, np, d, steps, cfg=1,
     us = ms / max(cnt, 1) * 1e3
     nbytes = 16 * mtxd * nanl + 24 * mtxd
     gbs = nbytes / (us * 1e-6) / 1e9
+    # HBM bytes per launch measured with rocprofv3 on this shape, when recorded
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "roofline_traffic.json")) as f:
+            e = json.load(f).get("kernels", {}).get(f"fp64/proj_kb/{mtxd}x{nanl}")
+        traffic = e["bytes"] if e else None
+    except (OSError, ValueError):
+        pass
     return {"workload": workload, "nanl": nanl, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
             "us_per_kernel": round(us, 2), "ms_per_call": round(ms_call, 3),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(gbs / PEAK_HBM_GBS, 3), "bytes_per_launch": nbytes}}
+                         "frac": round(gbs / PEAK_HBM_GBS, 3), "traffic": traffic, "bytes_per_launch": nbytes}}
 
 
 def bench_cfg2(torch, hk, synth, np, steps):
