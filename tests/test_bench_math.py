@@ -89,3 +89,15 @@ def test_stored_dpp_counts_follow_the_kernel_structure(key, shape, words):
         stored = json.load(f)["valu"][key]
     assert stored["dpp"] == want
     assert stored["dpp"] <= stored["valu"]
+
+
+def test_projector_traffic_record_is_the_write_stream():
+    """The recorded HBM traffic of k_proj_kb on the 64-atom cell (rocprofv3
+    FETCH_SIZE / WRITE_SIZE scaled by the copy calibration) is the projectors
+    written once plus little else: within 10% of 16 mtxd nanl + 24 mtxd."""
+    with open(os.path.join(ROOT, "profiles", "roofline_traffic.json")) as f:
+        rec = json.load(f)
+    e = rec["kernels"]["fp64/proj_kb/23847x256"]
+    algo = 16 * INFO["mtxd"] * NANL + 24 * INFO["mtxd"]
+    assert algo <= e["bytes"] <= 1.1 * algo
+    assert abs(e["bytes"] - (e["fetch_kib"] * rec["fetch_scale"] + e["write_kib"] * rec["write_scale"]) * 1024) < 2048
