@@ -202,6 +202,50 @@ def cpu_baseline(d, seconds, nb=8):
     return out
 
 
+def projector_cpu_baseline(d, pp, seconds):
+    """The reference proj_nl_kb_c16 (flang build, one core) when built, else the
+    numpy restatement, on the same inputs as the device build; bounded sample."""
+    import ctypes as C
+
+    import numpy as np
+
+    from oracle import oracle as O
+    from oracle import proj_kb as PK
+
+    m, nanl = int(d["mtxd"]), int(pp["nanl"])
+    kgv = np.asfortranarray(d["kgv"], np.int32)
+    isort = np.ascontiguousarray(d["isort"], np.int32)
+    rk = np.ascontiguousarray(d["kpt"], np.float64)
+    anl = np.zeros((m, nanl), complex, order="F")
+    xnl = np.zeros(nanl)
+    vkb = pp["vkb"]
+    if O.have_ref():
+        lib = O.ref_lib()
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        i = lambda v: C.byref(C.c_int(int(v)))
+        kind = "reference"
+        fn = lambda: lib.proj_nl_kb_c16_(
+            p(rk), i(m), p(isort), i(nanl), i(d["ng"]), p(kgv), p(pp["nqnl"]), p(pp["delqnl"]), p(vkb),
+            p(pp["nkb"]), i(pp["ntype"]), p(pp["natom"]), p(pp["rat"]), p(pp["adot"]), p(anl), p(xnl),
+            i(vkb.shape[3]), i(pp["rat"].shape[1]), i(vkb.shape[0] - 3), i(m), i(nanl), i(kgv.shape[1]))
+    else:
+        kind = "port"
+        fn = lambda: PK.proj_nl_kb_c16(rk, m, isort, nanl, d["ng"], kgv, pp["nqnl"], pp["delqnl"], vkb,
+                                       pp["nkb"], pp["ntype"], pp["natom"], pp["rat"], pp["adot"], m, nanl,
+                                       kgv.shape[1])
+    fn()
+    t0 = time.perf_counter()
+    calls = 0
+    while True:
+        fn()
+        calls += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return {"ms_per_call": round(dt / calls * 1e3, 2), "cores": 1, "kind": kind,
+            "sample": f"{calls} calls on the same inputs ({dt:.1f} s)"}
+
+
 def bench_kpoints(torch, hk, synth, np, steps):
     """16 k-points of the 64-atom cell on a shifted 2x2x4 grid (uneven
     plane-wave counts), 128 bands each: the eager k loop against one launch of
@@ -367,7 +411,8 @@ def bench_forces(torch, hk, synth, np, steps):
 
 
 def bench_projectors(torch, hk, synth, np, d, steps, cfg=1,
-                     workload="Si64 L=20.52, Ecut 15 Ha: 256 s+p KB projectors on 23,847 plane waves"):
+                     workload="Si64 L=20.52, Ecut 15 Ha: 256 s+p KB projectors on 23,847 plane waves",
+                     cpu_seconds=None):
     """KB projectors built on the device (cpwgpu_plan_build_projectors_kb, the
     construction of Src/omp/proj_nl_kb_c16.f90) for the cell of ``d``
     (default the 64-atom Si cell: 256 s and p projectors on 23,847 plane
@@ -404,7 +449,8 @@ def bench_projectors(torch, hk, synth, np, d, steps, cfg=1,
         traffic = e["bytes"] if e else None
     except (OSError, ValueError):
         pass
-    return {"workload": workload, "nanl": nanl, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
+    cpu = projector_cpu_baseline(d, pp, cpu_seconds) if cpu_seconds else None
+    return {"workload": workload, "nanl": nanl, "cpu_baseline": cpu, "mtxd": mtxd, "table_points": int(pp["nqnl"][0]), "launches": int(cnt),
             "us_per_kernel": round(us, 2), "ms_per_call": round(ms_call, 3),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(gbs / PEAK_HBM_GBS, 3), "traffic": traffic, "bytes_per_launch": nbytes}}
@@ -613,7 +659,8 @@ def main():
         steps = max(5, args.steps // 3)
         sec = {"spinor": lambda: bench_spin_density(torch, hk, synth, np, d, steps),
                "forces": lambda: bench_forces(torch, hk, synth, np, 5),
-               "projectors": lambda: bench_projectors(torch, hk, synth, np, d, max(10, args.steps)),
+               "projectors": lambda: bench_projectors(torch, hk, synth, np, d, max(10, args.steps),
+                                                      cpu_seconds=None if args.no_cpu else 3.0),
                "kpoints": lambda: bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))}[args.only]
         print(json.dumps({args.only: sec()}), flush=True)
         plan.close()
@@ -726,7 +773,8 @@ def main():
         log("bench: atomic forces (Si/Ge 64 atoms, 128 bands)")
         out["forces"] = bench_forces(torch, hk, synth, np, 5)
         log("bench: KB projectors built on the device (64-atom Si)")
-        out["projectors_kb"] = bench_projectors(torch, hk, synth, np, d, 20)
+        out["projectors_kb"] = bench_projectors(torch, hk, synth, np, d, 20,
+                                                cpu_seconds=None if args.no_cpu else 3.0)
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
