@@ -9,7 +9,8 @@ max |anlga|:
 * the same against the reference's own proj_nl_kb_c16 output (golden
   vectors of tests/golden/make_golden.py, two fcc cases);
 * the Fortran shim proj_nl_kb_c16_ (libcpwgpu_fortran.so), by reference;
-* the 64-atom Si cell (23,847 plane waves, 256 projectors) built into a plan:
+* the 64-atom Si cell (23,847 plane waves, 256 projectors) built into a plan
+  at k = 0 and at a k-point of the shifted 2x2x4 grid:
   the projectors equal the oracle's, and H|psi> with them equals H|psi>
   with the oracle's projectors uploaded from the host;
 * the non-local forces of the two-species Si/Ge cell with projectors built
@@ -215,4 +216,36 @@ def test_plan_build_rejects_bad_input_and_keeps_projectors(hk):
     plan.apply(psi, h1, hk.PREC_FP64)
     torch.cuda.synchronize()
     assert torch.equal(h0, h1)
+    plan.close()
+
+
+def test_plan_build_off_gamma_kpoint(hk):
+    """A k-point of the shifted 2x2x4 grid of the 64-atom cell (q = k + G in
+    the radial and angular parts, the phase from G alone): H|psi> with the
+    device-built projectors equals H|psi> with the oracle's, <= 1e-12."""
+    import torch
+
+    from cpw2000_amd import synth
+
+    d = synth.make_config(3, nbands=4, kindex=5)
+    rk = np.asarray(d["kpt"], float)
+    assert np.abs(rk).max() > 0.1
+    pp = synth.kb_pseudo(synth.config_spec(3))
+    kgv = np.asfortranarray(d["kgv"], np.int32)
+    m = d["mtxd"]
+    ref, xref = orc_kb.proj_nl_kb_c16(rk, m, d["isort"], pp["nanl"], d["ng"], kgv, pp["nqnl"], pp["delqnl"],
+                                      pp["vkb"], pp["nkb"], pp["ntype"], pp["natom"], pp["rat"], pp["adot"], m,
+                                      pp["nanl"], kgv.shape[1])
+    plan = hk.Plan.from_inputs(d, projectors=False)
+    plan.build_projectors_kb(rk, pp["nanl"], pp["nqnl"], pp["delqnl"], pp["vkb"], pp["nkb"], pp["ntype"],
+                             pp["natom"], pp["rat"], pp["adot"])
+    psi = torch.from_numpy(np.ascontiguousarray(d["psi"][:m].T)).to("cuda")
+    h_dev = torch.zeros_like(psi)
+    plan.apply(psi, h_dev, hk.PREC_FP64)
+    torch.cuda.synchronize()
+    plan.set_projectors(ref, xref)
+    h_ref = torch.zeros_like(psi)
+    plan.apply(psi, h_ref, hk.PREC_FP64)
+    torch.cuda.synchronize()
+    assert float(torch.linalg.norm(h_dev - h_ref) / torch.linalg.norm(h_ref)) <= TOL
     plan.close()
