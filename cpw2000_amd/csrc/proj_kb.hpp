@@ -72,8 +72,9 @@ constexpr int KB_THREADS = 256;
 
 // grid (ceil(mtxd / KB_THREADS), number of atoms); gw[j * mtxd + i] = g_j of
 // plane wave i; vkb packed per (type, l) line, line[j + 2] = vkb(j).  Every
-// table read has index nint(|q|/delq) + {-1, 0, 1} + 2 in 1..nq, inside the
-// line when nq <= vstride - 2 (checked on the host).
+// table read is at line position nint(|q|/delq) + {-1, 0, 1} + 2 with
+// nint(|q|/delq) in 0..nq - 2, so in 1..nq + 1: inside the line (vstride
+// values) when nq <= vstride - 2 (checked on the host).
 __global__ __launch_bounds__(KB_THREADS) void k_proj_kb(const double *__restrict__ gw, int mtxd, KbCell cell,
                                                         const KbAtom *__restrict__ atoms,
                                                         const KbType *__restrict__ types,
