@@ -41,6 +41,11 @@ CASES = {
     "ortho": (dict(cfg="ortho"), 8, True),
     "cfg1_b2": (dict(cfg=1), 2, True),
 }
+# large cases: only every ROW_STRIDE-th plane wave of the output is stored
+LARGE_CASES = {
+    "cfg2_b2": (dict(cfg=2), 2, True),
+}
+ROW_STRIDE = 4
 
 
 def digest(d):
@@ -55,7 +60,7 @@ def digest(d):
 
 
 def inputs(name):
-    kw, nb, proj = CASES[name]
+    kw, nb, proj = {**CASES, **LARGE_CASES}[name]
     d = synth.make_config(**kw)
     if not proj:
         d = dict(d)
@@ -123,10 +128,116 @@ def make_projkb_fixture():
     np.savez_compressed(os.path.join(HERE, "projkb_fcc2.npz"), meta=json.dumps(meta), **out)
 
 
-def main():
+# two-component case: hk_psi_spin_c16 on the 2-atom cell (24^3) with the
+# inputs of synth.spin_inputs (random sigma_z / sigma_x / sigma_y potentials,
+# spin-mixing projectors), 6 two-component bands
+SPIN_BANDS, SPIN_SEED = 6, 21
+
+
+def make_spin_fixture():
+    """Reference hk_psi_spin_c16 for nsp = 1, 2 and 4."""
+    lib = O.ref_lib()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    i = lambda v: C.byref(C.c_int(int(v)))
+    d = synth.make_config(0, nbands=2)
+    sp = synth.spin_inputs(d, SPIN_BANDS, seed=SPIN_SEED)
+    kgv = np.asfortranarray(d["kgv"], np.int32)
+    out = {}
+    for nsp in (1, 2, 4):
+        psi = np.asfortranarray(sp["psi_sp"])
+        h = np.zeros_like(psi, order="F")
+        lnew = C.c_int(1)
+        lib.hk_psi_spin_c16_(i(d["mtxd"]), i(SPIN_BANDS), p(psi), p(h), C.byref(lnew), i(d["ng"]), p(kgv),
+                             p(np.ascontiguousarray(d["ekpg"])), p(np.ascontiguousarray(d["isort"], np.int32)),
+                             p(np.asfortranarray(sp["vscr_sp"])), p(np.ascontiguousarray(d["kmscr"], np.int32)),
+                             i(nsp), p(np.asfortranarray(sp["anlsp"])), p(np.ascontiguousarray(sp["xnlkbsp"])),
+                             i(sp["nanlsp"]), i(d["mxddim"]), i(SPIN_BANDS), i(sp["mxdasp"]), i(d["mxdgve"]),
+                             i(d["mxdscr"]), i(sp["mxdnsp"]))
+        out[f"hpsi_nsp{nsp}"] = h[: 2 * d["mtxd"]]
+        print("hk_psi_spin_c16 nsp", nsp, "|hpsi| =", np.linalg.norm(h))
+    w = np.cos(0.37 * np.arange(sp["vscr_sp"].size))
+    meta = dict(nbands=SPIN_BANDS, seed=SPIN_SEED, digest=digest(d),
+                digest_vsp=float(np.dot(w, sp["vscr_sp"].reshape(-1, order="F"))),
+                digest_psi=[float(np.real(np.sum(sp["psi_sp"] * np.cos(np.arange(sp["psi_sp"].shape[0]))[:, None]))),
+                            float(np.imag(np.sum(sp["psi_sp"] * np.cos(np.arange(sp["psi_sp"].shape[0]))[:, None])))])
+    np.savez_compressed(os.path.join(HERE, "hkpsi_spin_cfg0.npz"), meta=json.dumps(meta), **out)
+
+
+# forces: the Si/Ge 64-atom cell, FORCE_BANDS bands with occupations
+# FORCE_OCC, the s+p KB tables of synth.kb_pseudo
+FORCE_BANDS = 32
+
+
+def force_occ(nb=FORCE_BANDS):
+    return np.linspace(2.0, 0.5, nb)
+
+
+def make_forces_fixture():
+    """The reference's force routines on the Si/Ge cell.
+
+    * for_str_nl_kb (non-local, covariant, per unit of the fractional
+      coordinate) with the KB tables of synth.kb_pseudo, the first
+      FORCE_BANDS bands and occupations force_occ();
+    * for_str_local_force with one star per q vector of the local form
+      factors (mstar = 1, phase = 1, conj = 1; star 1 is q = 0), den(q) the
+      band density's Fourier coefficients (numpy FFT of |psi(r)|^2, the
+      convention rho(q) = (1/N) sum_r rho(r) exp(-i q.r)), vql = the form
+      factors, no core charge."""
+    from oracle import forces as Fo
+
+    lib = O.ref_lib()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    i = lambda v: C.byref(C.c_int(int(v)))
+    d = synth.make_config(4)
+    spec = synth.config_spec(4)
+    pp = synth.kb_pseudo(spec)
+    nb = FORCE_BANDS
+    psi = np.asfortranarray(d["psi"][:, :nb])
+    occ = np.ascontiguousarray(force_occ(nb))
+    ntype, mxdatm = int(pp["ntype"]), pp["rat"].shape[1]
+    vkb = np.asfortranarray(pp["vkb"])
+    kgv = np.asfortranarray(d["kgv"], np.int32)
+    fnl = np.zeros((3, mxdatm, ntype), order="F")
+    snl = np.zeros((3, 3), order="F")
+    lib.for_str_nl_kb_(p(fnl), p(snl), i(d["mtxd"]), p(np.ascontiguousarray(d["kpt"], float)), i(nb), p(occ),
+                       p(np.ascontiguousarray(d["isort"], np.int32)), p(psi), p(kgv), p(pp["nqnl"]), p(pp["delqnl"]),
+                       p(vkb), p(np.asfortranarray(pp["nkb"], np.int32)), i(ntype), p(pp["natom"]),
+                       p(np.asfortranarray(pp["rat"])), p(np.asfortranarray(pp["adot"])), i(ntype), i(mxdatm),
+                       i(kgv.shape[1]), i(vkb.shape[0] - 3), i(d["mxddim"]), i(nb))
+    # local term: one star per q (star 1 = q 0)
+    q = np.asarray(d["floc_q"], np.int64)
+    keep = np.any(q != 0, axis=1)
+    q, fq = q[keep], np.asarray(d["floc_f"])[keep]
+    n1, n2, n3 = [int(x) for x in d["kmscr"][3:6]]
+    rq = Fo.band_density_q(d, psi, occ)
+    den = np.concatenate([[0.0], rq[q[:, 2] % n3, q[:, 1] % n2, q[:, 0] % n1]]).astype(complex)
+    ns = q.shape[0] + 1
+    kloc = np.asfortranarray(np.concatenate([np.zeros((1, 3), np.int64), q]).T.astype(np.int32))
+    vql = np.asfortranarray(np.concatenate([np.zeros((1, ntype)), fq]).T)
+    phase = np.ones(ns, complex)
+    conj = np.ones(ns)
+    mstar = np.ones(ns, np.int32)
+    vxc = np.zeros(ns, complex)
+    dnc = np.zeros((ntype, ns), order="F")
+    floc = np.zeros((3, mxdatm, ntype), order="F")
+    lib.for_str_local_force_(p(floc), i(ns), p(kloc), p(phase), p(conj), i(ns), p(mstar), p(vxc), p(den),
+                             i(ntype), p(pp["natom"]), p(np.asfortranarray(pp["rat"])), p(vql), p(dnc), i(ns),
+                             i(ns), i(ntype), i(mxdatm))
+    natom = pp["natom"]
+    glob = lambda f: np.concatenate([f[:, :natom[t], t].T for t in range(ntype)])
+    meta = dict(nbands=nb, digest=digest(d), nq=int(q.shape[0]))
+    np.savez_compressed(os.path.join(HERE, "forces_sige64.npz"), meta=json.dumps(meta), fnl_cov=glob(fnl),
+                        floc_cov=glob(floc), occ=occ, den=den[1:])
+    print("for_str_nl_kb |f| =", np.abs(fnl).max(), " for_str_local_force |f| =", np.abs(floc).max())
+
+
+def main(only=None):
+    """Writes every fixture, or only the hk_psi cases named in ``only``."""
     if not O.build_ref():
         raise SystemExit("reference sources not found: cannot write golden vectors")
     for name in CASES:
+        if only and name not in only:
+            continue
         d, psi = inputs(name)
         hpsi = O.ref_hk_psi(d, psi=psi)
         kw, nb, proj = CASES[name]
@@ -135,9 +246,27 @@ def main():
                                                  digest=digest(d))),
                             hpsi=hpsi[: d["mtxd"]])
         print(name, d["mtxd"], nb, "|hpsi| =", np.linalg.norm(hpsi))
+    for name, (kw, nb, proj) in LARGE_CASES.items():
+        if only and name not in only:
+            continue
+        d, psi = inputs(name)
+        hpsi = O.ref_hk_psi(d, psi=psi)
+        np.savez_compressed(os.path.join(HERE, f"hkpsi_{name}.npz"),
+                            meta=json.dumps(dict(kwargs=kw, nbands=nb, projectors=proj, row_stride=ROW_STRIDE,
+                                                 digest=digest(d))),
+                            hpsi=hpsi[: d["mtxd"]:ROW_STRIDE])
+        print(name, d["mtxd"], nb, "|hpsi| =", np.linalg.norm(hpsi))
+    if only:
+        if "spin" in only:
+            make_spin_fixture()
+        if "forces" in only:
+            make_forces_fixture()
+        return
     make_fft_fixture()
     make_projkb_fixture()
+    make_spin_fixture()
+    make_forces_fixture()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -39,3 +39,38 @@ def test_nonlocal_energy_matches_operator(orc):
     d0 = dict(d, nanl=0)
     loc = Fo.potential_energy(d0, psi, lambda dd, p: orc.orc_hk_psi(dd, psi=p)[1])
     assert abs((full - loc) - enl) < 1e-12 * max(1.0, abs(enl))
+
+
+def test_force_oracle_matches_reference_routines():
+    """The restatement against cpw2000's own for_str_nl_kb and
+    for_str_local_force (golden vectors of the flang build, Si/Ge 64 atoms,
+    32 bands with occupations; non-local with the projectors of the KB tables
+    by the proj_nl_kb_c16 restatement, local with the stored band density),
+    covariant components, to 1e-12 of the largest."""
+    import json
+    import os
+
+    from oracle import forces as Fo
+    from oracle import proj_kb as PK
+    from tests.conftest import GOLDEN
+
+    z = np.load(os.path.join(GOLDEN, "forces_sige64.npz"), allow_pickle=False)
+    nb = json.loads(str(z["meta"]))["nbands"]
+    d = synth.make_config(4)
+    psi = np.asfortranarray(d["psi"][:, :nb])
+    occ = z["occ"]
+    pp = synth.kb_pseudo(synth.config_spec(4))
+    kgv = np.asfortranarray(d["kgv"], np.int32)
+    a, x = PK.proj_nl_kb_c16(np.asarray(d["kpt"], float), d["mtxd"], d["isort"], pp["nanl"], d["ng"], kgv,
+                             pp["nqnl"], pp["delqnl"], pp["vkb"], pp["nkb"], pp["ntype"], pp["natom"], pp["rat"],
+                             pp["adot"], d["mxddim"], pp["nanl"], kgv.shape[1])
+    fnl, _ = Fo.nl_forces_cov(dict(d, anlga=a, xnlkb=x), psi, occ)
+    assert np.abs(fnl - z["fnl_cov"]).max() < 1e-12 * np.abs(z["fnl_cov"]).max()
+    # the stored density is the oracle's own band density (its FFT is pinned elsewhere)
+    n1, n2, n3 = [int(v) for v in d["kmscr"][3:6]]
+    q = np.asarray(d["floc_q"], np.int64)
+    q = q[np.any(q != 0, axis=1)]
+    rq = Fo.band_density_q(d, psi, occ)
+    assert np.abs(rq[q[:, 2] % n3, q[:, 1] % n2, q[:, 0] % n1] - z["den"]).max() < 1e-14
+    fl = Fo.local_forces_cov(d, psi, occ)
+    assert np.abs(fl - z["floc_cov"]).max() < 1e-12 * np.abs(z["floc_cov"]).max()

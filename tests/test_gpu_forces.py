@@ -87,3 +87,54 @@ def test_autograd_gradcheck_two_atoms(torch_dev):
     tau = torch.from_numpy(d0["rat"] @ avec.T).requires_grad_(True)
     fn = lambda t: PotentialEnergy.apply(t, builder, psi_t)
     assert torch.autograd.gradcheck(fn, (tau,), eps=1e-4, atol=1e-6, rtol=1e-4)
+
+
+def test_forces_match_reference_routines(torch_dev):
+    """Against cpw2000's own force routines (flang build of the reference,
+    golden vectors tests/golden/forces_sige64.npz): for_str_nl_kb for the
+    non-local term with the projectors built on the device from the same KB
+    tables, for_str_local_force for the local term (one star per q of the
+    form factors, the band density as den), 32 bands with occupations 2 ..
+    0.5.  The reference's covariant components (per unit of the fractional
+    coordinate) become Cartesian through inv(avec)^T, the step of
+    Src/cpw_lib/force_stress_kb.f90:163-167 (adotm1 = bdot / 4 pi^2) followed
+    by avec.  Tolerance 1e-9 Ha/bohr."""
+    import json
+    import os
+
+    from cpw2000_amd import hk_psi
+    from tests.conftest import GOLDEN
+    from tests.golden.make_golden import digest
+
+    torch = torch_dev
+    z = np.load(os.path.join(GOLDEN, "forces_sige64.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    d = synth.make_config(4)
+    dg = digest(d)
+    for k, v in meta["digest"].items():
+        np.testing.assert_allclose(dg[k], v, rtol=1e-12, atol=1e-12, err_msg=f"generator drifted ({k})")
+    nb, occ = meta["nbands"], z["occ"]
+    psi = _psi_t(torch, d)[:nb]
+    to_cart = lambda f: f @ np.linalg.inv(d["avec"])
+    # non-local term, projectors from the KB tables on the device
+    pp = synth.kb_pseudo(synth.config_spec(4))
+    plan = hk_psi.Plan.from_inputs(d, projectors=False)
+    plan.build_projectors_kb(np.asarray(d["kpt"], float), pp["nanl"], pp["nqnl"], pp["delqnl"], pp["vkb"],
+                             pp["nkb"], pp["ntype"], pp["natom"], pp["rat"], pp["adot"])
+    fnl, _ = plan.forces(psi, d["rat"], d["avec"], d["proj_atom"], occ=occ)
+    plan.close()
+    ref = to_cart(z["fnl_cov"])
+    err = np.abs(fnl - ref).max()
+    print(f"non-local forces vs for_str_nl_kb: max |dF| {err:.2e} of max |F| {np.abs(ref).max():.2e} Ha/bohr")
+    assert np.abs(ref).max() > 1e-5
+    assert err < 1e-9
+    # local term alone (no projectors on this plan)
+    plan = hk_psi.Plan.from_inputs(d, projectors=False)
+    floc, _ = plan.forces(psi, d["rat"], d["avec"], np.zeros(0, np.int32), d["floc_q"], d["floc_f"],
+                          d["species"], occ=occ)
+    plan.close()
+    ref = to_cart(z["floc_cov"])
+    err = np.abs(floc - ref).max()
+    print(f"local forces vs for_str_local_force: max |dF| {err:.2e} of max |F| {np.abs(ref).max():.2e} Ha/bohr")
+    assert np.abs(ref).max() > 1e-3
+    assert err < 1e-9

@@ -169,3 +169,51 @@ def test_fortran_shim_hk_psi_spin_c16(hk):
                          i(d["mxdscr"]), i(s["mxdnsp"]))
     assert lnew.value == 1
     assert _rel(hpsi[: 2 * d["mtxd"]], ref) < TOL64
+
+
+def _spin_golden():
+    """Inputs and the reference's own hk_psi_spin_c16 outputs (flang build of
+    Src/hk_psi/reference/hk_psi_spin_c16.f90, nsp = 1, 2, 4; golden vectors
+    tests/golden/hkpsi_spin_cfg0.npz)."""
+    import json
+    import os
+
+    from tests.conftest import GOLDEN
+    from tests.golden.make_golden import digest
+
+    z = np.load(os.path.join(GOLDEN, "hkpsi_spin_cfg0.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    d = synth.make_config(0, nbands=2)
+    dg = digest(d)
+    for k, v in meta["digest"].items():
+        np.testing.assert_allclose(dg[k], v, rtol=1e-12, atol=1e-12, err_msg=f"generator drifted ({k})")
+    s = synth.spin_inputs(d, meta["nbands"], seed=meta["seed"])
+    return d, s, meta, z
+
+
+@pytest.mark.parametrize("nsp", [1, 2, 4])
+def test_spin_matches_reference_golden(torch_dev, hk, nsp):
+    """Device plan (every coupled kernel for nsp = 4) and host drop-in against
+    the reference's output, FP64 to 1e-12 and FP32-mixed to 2e-6."""
+    torch = torch_dev
+    d, s, meta, z = _spin_golden()
+    ref = z[f"hpsi_nsp{nsp}"]
+    mtxd, nb = d["mtxd"], meta["nbands"]
+    plan = _plan(hk, d, s, nsp)
+    pt = torch.from_numpy(np.ascontiguousarray(s["psi_sp"][: 2 * mtxd].T)).to("cuda")
+    for kern in _spin_kernels(hk, plan, nsp):
+        for prec, tol in ((hk.PREC_FP64, TOL64), (hk.PREC_FP32, TOL32)):
+            plan.set_variant(prec, hk.TUNE_SPIN, kern)
+            ht = torch.zeros_like(pt)
+            plan.apply_spin(pt, ht, prec)
+            torch.cuda.synchronize()
+            assert _rel(ht.cpu().numpy().T, ref) < tol, (nsp, kern, prec)
+    plan.close()
+    hpsi = np.zeros_like(s["psi_sp"], order="F")
+    hk.hk_psi_spin_c16(mtxd, nb, s["psi_sp"], hpsi, True, d["ng"], np.asfortranarray(d["kgv"], np.int32),
+                       d["ekpg"], np.asfortranarray(d["isort"], np.int32), s["vscr_sp"],
+                       np.asfortranarray(d["kmscr"], np.int32), nsp, s["anlsp"], s["xnlkbsp"], s["nanlsp"],
+                       d["mxddim"], nb, s["mxdasp"], d["mxdgve"], d["mxdscr"], s["mxdnsp"])
+    err = _rel(hpsi[: 2 * mtxd], ref)
+    print(f"hk_psi_spin_c16 nsp {nsp}: drop-in FP64 vs reference {err:.2e}")
+    assert err < TOL64

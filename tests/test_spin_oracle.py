@@ -95,3 +95,24 @@ def test_noncollinear_operator_is_hermitian(case):
     h = spin.hk_psi_spin(d, s["psi_sp"], *args)
     g = psi.conj().T @ h
     assert np.abs(g - g.conj().T).max() < 1e-12 * np.abs(g).max()
+
+
+@pytest.mark.parametrize("nsp", [1, 2, 4])
+def test_spin_oracle_matches_reference_golden(nsp):
+    """Against the reference's own hk_psi_spin_c16 (flang build, golden
+    vectors tests/golden/hkpsi_spin_cfg0.npz, 6 two-component bands of the
+    2-atom cell with spin-mixing projectors), to 1e-13."""
+    import json
+    import os
+
+    from tests.conftest import GOLDEN
+    from tests.golden.make_golden import digest
+
+    z = np.load(os.path.join(GOLDEN, "hkpsi_spin_cfg0.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    d = synth.make_config(0, nbands=2)
+    for k, v in meta["digest"].items():
+        np.testing.assert_allclose(digest(d)[k], v, rtol=1e-12, atol=1e-12)
+    s = synth.spin_inputs(d, meta["nbands"], seed=meta["seed"])
+    h = spin.hk_psi_spin(d, s["psi_sp"], s["vscr_sp"], nsp, s["anlsp"], s["xnlkbsp"], s["nanlsp"])
+    assert _rel(h[: 2 * d["mtxd"]], z[f"hpsi_nsp{nsp}"]) < 1e-13
