@@ -189,16 +189,30 @@ def cpu_baseline(d, seconds, nb=8):
             break
     out = {"value": round(calls * nb / dt, 2), "unit": "bands/s", "cores": cores, "kind": kind,
            "sample": f"{calls} calls x {nb} bands of the same 64-atom Si workload ({dt:.1f} s)"}
-    # the C restatement with OpenMP on the host cores, for scale
-    port_threads = O.orc_lib().orc_num_threads()
-    O.orc_hk_psi(d, psi=psi)
-    t0 = time.perf_counter()
-    calls = 0
-    while time.perf_counter() - t0 < seconds / 2:
-        O.orc_hk_psi(d, psi=psi)
-        calls += 1
-    dt = time.perf_counter() - t0
-    out["port_openmp"] = {"value": round(calls * nb / dt, 2), "cores": port_threads}
+
+    def port_rate(threads):
+        lib = O.orc_lib()
+        prev = lib.orc_num_threads()
+        lib.orc_set_num_threads(threads)
+        try:
+            O.orc_hk_psi(d, psi=psi)
+            t0 = time.perf_counter()
+            n = 0
+            while time.perf_counter() - t0 < seconds / 2:
+                O.orc_hk_psi(d, psi=psi)
+                n += 1
+            return round(n * nb / (time.perf_counter() - t0), 2), lib.orc_num_threads()
+        finally:
+            lib.orc_set_num_threads(prev)
+
+    # the C restatement on one core (calibrates it against the reference on
+    # the same core) and with OpenMP on the host cores
+    v1, _ = port_rate(1)
+    out["port_1core"] = {"value": v1, "cores": 1}
+    if kind == "reference":
+        out["reference_over_port_1core"] = round(out["value"] / v1, 3)
+    vn, nth = port_rate(O.orc_lib().orc_num_threads())
+    out["port_openmp"] = {"value": vn, "cores": nth}
     return out
 
 
@@ -339,7 +353,64 @@ def bench_ditsp(torch, hk, plan, d, nb, np):
     out["eig_fp32_vs_fp64_ha"] = float(np.abs(eis["fp64"] - eis["fp32"]).max())
     out["bands"] = nb
     out["epspsi"] = 1e-6
+    try:
+        from cpw2000_amd import synth
+
+        out["cpu_baseline"] = ditsp_cpu_baseline(hk, synth, np)
+    except (OSError, RuntimeError) as exc:
+        log(f"bench: reference ditsp unavailable ({exc})")
+        out["cpu_baseline"] = None
     return out
+
+
+def ditsp_cpu_baseline(hk, synth, np, cfg=0, neig=8, epspsi=1e-6):
+    """The reference ditsp_c16 (flang build of Src/ditsp with its vendored
+    BLAS / LAPACK and the reference hk_psi_c16, one core) against the device
+    drop-in cpwgpu_ditsp_c16 on the same host arrays: neig bands of the
+    2-atom cell to max residual^2 < epspsi^2 from the same small matrix
+    (leading block of H, built here by applying H to unit vectors) and diag(H).
+    Wall time of each call and the largest eigenvalue difference."""
+    from oracle import oracle as O
+
+    if not O.have_ref():
+        return None
+    d = synth.make_config(cfg, nbands=2)
+    m, mxddim = d["mtxd"], d["mxddim"]
+    ms = 3 * neig + 20
+
+    def args():
+        psi = np.zeros((mxddim, neig), np.complex128, order="F")
+        return dict(ipr=0, icmax=1000, iguess=0, epspsi=epspsi, lnewanl=True, neig=neig, mtxd=m, mtxds=ms,
+                    mxdsml=ms, mxdbnd=neig, psi=psi, hpsi=np.zeros_like(psi, order="F"), ei=np.zeros(neig),
+                    ekpg=np.asfortranarray(d["ekpg"]), isort=np.asfortranarray(d["isort"], np.int32),
+                    vscr=np.asfortranarray(d["vscr"]), kmscr=np.asfortranarray(d["kmscr"], np.int32), ng=d["ng"],
+                    kgv=np.asfortranarray(d["kgv"], np.int32), anlga=np.asfortranarray(d["anlga"], np.complex128),
+                    xnlkb=np.asfortranarray(d["xnlkb"]), nanl=d["nanl"], hamsm=hs, hdiag=hd, mxddim=mxddim,
+                    mxdgve=d["mxdgve"], mxdscr=d["mxdscr"], mxdanl=max(d["mxdanl"], 1))
+
+    unit = np.zeros((mxddim, ms), np.complex128, order="F")
+    unit[np.arange(ms), np.arange(ms)] = 1.0
+    _, he = O.orc_hk_psi(d, psi=unit)
+    hs = np.asfortranarray(he[:ms, :ms])
+    n = int(np.prod(d["kmscr"][3:6]))
+    hd = np.zeros(mxddim, order="F")
+    hd[:m] = (d["ekpg"][:m] + np.mean(d["vscr"][:n])
+              + (np.abs(d["anlga"][:m, :d["nanl"]]) ** 2) @ d["xnlkb"][:d["nanl"]])
+    a = args()
+    t0 = time.perf_counter()
+    ifail_ref = O.ref_ditsp_c16(a)
+    t_ref = time.perf_counter() - t0
+    hk.set_precision(hk.PREC_FP64)
+    b = args()
+    hk.ditsp_c16(**{k: v for k, v in b.items()})  # warm-up (plan, handles)
+    b = args()
+    t0 = time.perf_counter()
+    ifail_dev = hk.ditsp_c16(**b)
+    t_dev = time.perf_counter() - t0
+    return {"workload": f"2-atom Si, {m} plane waves, {neig} bands, epspsi {epspsi}",
+            "reference_s": round(t_ref, 4), "reference_ifail": ifail_ref, "cores": 1, "kind": "reference",
+            "device_dropin_s": round(t_dev, 4), "device_ifail": ifail_dev,
+            "eig_diff_ha": float(np.abs(a["ei"] - b["ei"]).max())}
 
 
 def bench_spin_density(torch, hk, synth, np, d, steps):
@@ -495,6 +566,8 @@ def bench_cfg2(torch, hk, synth, np, steps):
         m = (x.conj() @ h[prec].T).cpu().numpy()
         eig[prec] = sla.eigh(0.5 * (m + m.conj().T), s_mat, eigvals_only=True)
     out["eig_err_ha"] = float(np.abs(eig["fp32"] - eig["fp64"]).max())
+    out["eig_tol_ha"] = 1e-6
+    out["eig_ok"] = out["eig_err_ha"] <= 1e-6
     plan.close()
     return out
 
@@ -604,15 +677,15 @@ def main():
     ap.add_argument("--no-kpoints", action="store_true", help="skip the 16 k-point graph section")
     ap.add_argument("--no-ditsp", action="store_true",
                     help="skip the iterative diagonalisation section")
-    ap.add_argument("--extra", action="store_true",
-                    help="also measure the 216-atom cell (108^3 mesh, 512 bands)")
+    ap.add_argument("--no-si216", action="store_true",
+                    help="skip the 216-atom cell (108^3 mesh, 512 bands)")
     ap.add_argument("--variant-sweep", action="store_true",
                     help="only time every projection / back-projection kernel alone (one "
                          "stream), then the calibration copy; prints their JSON and exits")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the 64-atom headline (tuning, accuracy, timed loops, rooflines): "
                          "for profiling its kernels")
-    ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints", "projectors"),
+    ap.add_argument("--only", choices=("ditsp", "spinor", "forces", "kpoints", "projectors", "si216"),
                     help="run only this section (after tuning the 64-atom plan) and print its JSON; "
                          "for profiling one path at a time")
     ap.add_argument("--issue-rates", action="store_true",
@@ -654,14 +727,15 @@ def main():
 
     # autotuner: plane-stage kernel and split-K target per precision, timed on
     # this workload (part of the warm-up; every later apply uses the choice)
-    if args.only in ("spinor", "forces", "kpoints", "projectors"):
+    if args.only in ("spinor", "forces", "kpoints", "projectors", "si216"):
         # these sections build and tune their own plans
         steps = max(5, args.steps // 3)
         sec = {"spinor": lambda: bench_spin_density(torch, hk, synth, np, d, steps),
                "forces": lambda: bench_forces(torch, hk, synth, np, 5),
                "projectors": lambda: bench_projectors(torch, hk, synth, np, d, max(10, args.steps),
                                                       cpu_seconds=None if args.no_cpu else 3.0),
-               "kpoints": lambda: bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6))}[args.only]
+               "kpoints": lambda: bench_kpoints(torch, hk, synth, np, max(3, args.steps // 6)),
+               "si216": lambda: bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))}[args.only]
         print(json.dumps({args.only: sec()}), flush=True)
         plan.close()
         return
@@ -778,7 +852,7 @@ def main():
     if not args.no_ditsp:
         log("bench: iterative diagonalisation on the device (128 bands)")
         out["ditsp"] = bench_ditsp(torch, hk, plan, d, nb, np)
-    if args.extra:
+    if not (args.no_si216 or args.headline_only):
         log("bench: 216-atom cell (108^3, 512 bands)")
         out["si216"] = bench_cfg2(torch, hk, synth, np, max(3, args.steps // 6))
     for key, prec in (("roofline", "fp32"), ("roofline_fp64", "fp64")):

@@ -23,6 +23,7 @@
 #include "fft_fast.hpp"
 #include "fft_local.hpp"
 #include "fft_reg.hpp"
+#include "fft_wave.hpp"
 #include "nl_gemm.hpp"
 #include "subspace.hpp"
 #include "forces.hpp"
@@ -420,7 +421,11 @@ template <> struct RegShape<double> {
     static constexpr int GX[3] = {8, 8, 4}, GY[3] = {8, 8, 8}, P[3] = {1, 1, 1};
     static constexpr int T[3] = {256, 320, 256}, W[3] = {3, 4, 3};
 };
-constexpr int NPLANE_VAR = 4;
+// 4..6: k_plane_wave (LDS two-pass lines, wave-local exchanges) with
+// 256, 192 and 128 threads per block; 7: register shape 1 for the x rows with
+// the y lines through LDS (wave-local exchanges)
+constexpr int NPLANE_VAR = 8;
+constexpr int WAVE_T[3] = {256, 192, 128};
 // back-projection kernels: 0 LDS-staged, 1..NBACK_DIRECT direct, NBACK_DIRECT+1 LDS-staged with half chunks
 constexpr int NBACK_VAR = NBACK_DIRECT + 2;
 // projection kernels: 0 LDS-staged, 1..NPROJ_DIRECT direct, NPROJ_DIRECT+1 LDS-staged with half wave tiles
@@ -438,7 +443,7 @@ template <typename R, int N, int GX, int GY> constexpr bool reg_ok() {
 // LDS row stride of the register variant's sub-plane for nx box columns
 inline int reg_stride(int nx) { return plane_sub_stride(nx); }
 
-template <typename R, int N, int V>
+template <typename R, int N, int V, bool YW = false>
 int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr,
                      hipStream_t s) {
     constexpr int GX = RegShape<R>::GX[V - 1], GY = RegShape<R>::GY[V - 1];
@@ -453,12 +458,12 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
             return fail(CPWGPU_ERR_SIZE, "plane variant %d: %zu bytes of LDS", V, dyn);
         static bool attr = false;
         if (!attr) {
-            HIPCHK(hipFuncSetAttribute((const void *)k_plane_reg<R, N, GX, GY, P, T, W>,
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_reg<R, N, GX, GY, P, T, W, YW>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
         ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W>), dim3(((a.nb + P - 1) / P) * g.n[2]),
+        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W, YW>), dim3(((a.nb + P - 1) / P) * g.n[2]),
                            dim3(T), dyn, s, colbuf, (const int *)p->col_uy.p, vr,
                            (const double2 *)p->tw[0].p, g, a.nb, S);
         HIPCHK(hipGetLastError());
@@ -466,10 +471,39 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
     }
 }
 
+template <typename R, int N, int T>
+int launch_plane_wave(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
+    const FftGeom &g = p->g;
+    const size_t dyn = plane_wave_lds<R, N, T>(g.nx);
+    const size_t stat = sizeof(cplx<R>) * 2 * N;
+    if (dyn + stat > 160 * 1024)
+        return fail(CPWGPU_ERR_SIZE, "plane of %d x %d does not fit in LDS with %d threads", N, N, T);
+    static bool attr = false;
+    if (!attr) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_plane_wave<R, N, N, T, 2>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)));
+        attr = true;
+    }
+    ProfScope ps_(CPWGPU_K_PLANE, s);
+    hipLaunchKernelGGL((k_plane_wave<R, N, N, T, 2>), dim3(g.n[2] * a.nb), dim3(T), dyn, s, colbuf,
+                       (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p, (const double2 *)p->tw[1].p, g);
+    HIPCHK(hipGetLastError());
+    return CPWGPU_OK;
+}
+
+// whether the wave-exchange variant v (4..6) fits in LDS for nx box columns
+template <typename R, int N> bool plane_wave_fits(int v, int nx) {
+    const size_t stat = sizeof(cplx<R>) * 2 * N;
+    const size_t dyn = v == 4 ? plane_wave_lds<R, N, WAVE_T[0]>(nx)
+                       : v == 5 ? plane_wave_lds<R, N, WAVE_T[1]>(nx) : plane_wave_lds<R, N, WAVE_T[2]>(nx);
+    return dyn + stat <= 160 * 1024;
+}
+
 template <typename R, int N> constexpr bool plane_var_ok(int v) {
     return v == 0 || (v == 1 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>()) ||
            (v == 2 && reg_ok<R, N, RegShape<R>::GX[1], RegShape<R>::GY[1]>()) ||
-           (v == 3 && reg_ok<R, N, RegShape<R>::GX[2], RegShape<R>::GY[2]>());
+           (v == 3 && reg_ok<R, N, RegShape<R>::GX[2], RegShape<R>::GY[2]>()) || (v >= 4 && v <= 6) ||
+           (v == 7 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>());
 }
 
 // default plane variant of a mesh length
@@ -482,6 +516,10 @@ int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R 
     if (v == 1) return launch_plane_reg<R, N, 1>(p, a, colbuf, vr, s);
     if (v == 2) return launch_plane_reg<R, N, 2>(p, a, colbuf, vr, s);
     if (v == 3) return launch_plane_reg<R, N, 3>(p, a, colbuf, vr, s);
+    if (v == 4) return launch_plane_wave<R, N, WAVE_T[0]>(p, a, colbuf, vr, s);
+    if (v == 5) return launch_plane_wave<R, N, WAVE_T[1]>(p, a, colbuf, vr, s);
+    if (v == 6) return launch_plane_wave<R, N, WAVE_T[2]>(p, a, colbuf, vr, s);
+    if (v == 7) return launch_plane_reg<R, N, 1, true>(p, a, colbuf, vr, s);
     return launch_fast<R, N>(p, a, colbuf, vr, ST_PLANE, s);
 }
 
@@ -503,7 +541,7 @@ template <typename R> int plane_var_resolve(const cpwgpu_plan_s *p, int v, bool 
 #define CPW_PV(N)                                      \
     if (g.n[0] == N) {                                 \
         def = plane_var_default<R, N>();               \
-        if (v > 0) *ok = v < NPLANE_VAR && plane_var_ok<R, N>(v); \
+        if (v > 0) *ok = v < NPLANE_VAR && plane_var_ok<R, N>(v) && (v < 4 || v > 6 || plane_wave_fits<R, N>(v, g.nx)); \
     }
     CPW_FAST_SIZES(CPW_PV)
 #undef CPW_PV
@@ -593,11 +631,61 @@ ProjView spin_proj(cpwgpu_plan_s *p) {
             (const double *)p->sxnl.p, 2 * p->g.mtxd};
 }
 
+// Caller-provided device workspace of one apply (the allocation-free entry
+// point): the z-column buffer of the local stage and the split-K partials and
+// D<beta|psi> of the non-local stage, each with its size in bytes.  Null:
+// the plan's own grow-only buffers.
+struct Work {
+    void *colbuf = nullptr;
+    size_t colbuf_n = 0;
+    void *part = nullptr;
+    size_t part_n = 0;
+    void *dhd = nullptr;
+    size_t dhd_n = 0;
+};
+
+// Block tile (BM x BN) and K granule of the projection kernel variant pv.
+template <typename R> void proj_tile(int pv, int *BM, int *BN, int *BK) {
+    *BM = proj_bm<R>();
+    *BN = proj_bn<R>();
+    *BK = GemmCfg<R>::BK;
+    if (pv == NPROJ_DIRECT + 1) {
+        *BM = 2 * proj_half_wm<R>() * 16;
+        *BN = 2 * proj_half_wn<R>() * 16;
+    }
+#define CPW_PD(V) if (pv == V) { *BM = direct_bm<R, V>(); *BN = direct_bn<R, V>(); *BK = direct_bk<R, V>(); }
+    CPW_PD(1) CPW_PD(2) CPW_PD(3) CPW_PD(4)
+#undef CPW_PD
+}
+
+// projection kernel variant of the plan for precision R
+template <typename R> int proj_variant(const cpwgpu_plan_s *p) {
+    const int ix = sizeof(R) == 4 ? 1 : 0;
+    return p->proj_var[ix] >= 1 && p->proj_var[ix] < NPROJ_VAR ? p->proj_var[ix] : 0;
+}
+
+// Split-K shape of the projection GEMM over rows plane waves: number of K
+// slices and rows per slice (whole LDS stages of k_proj / whole chunks of
+// k_proj_direct), for nanl projectors and neig bands.
+template <typename R> void splitk_shape(const cpwgpu_plan_s *p, int rows, int nanl, int neig, int *nsl_out,
+                                        int *kslice_out) {
+    int BM, BN, BK;
+    proj_tile<R>(proj_variant<R>(p), &BM, &BN, &BK);
+    const int ix = sizeof(R) == 4 ? 1 : 0;
+    const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
+    const int ksteps = (rows + BK - 1) / BK;
+    const int target = p->splitk[ix] > 0 ? p->splitk[ix] : 256;
+    int nsl = std::max(1, std::min(ksteps, (target + tm * tn - 1) / (tm * tn)));
+    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
+    *nsl_out = (rows + kslice - 1) / kslice;
+    *kslice_out = kslice;
+}
+
 // hpsi = beta*hpsi + sum_a beta_a xnlkb_a <beta_a|psi> (Src/cpw_lib/hk_psi_nl_c16.f90:82-96),
 // plus ekpg*psi when kin (the kinetic term, only for the plan's own rows)
 template <typename R>
 int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi, long ldp, double *hpsi,
-             long ldh, int beta, bool kin, hipStream_t s) {
+             long ldh, int beta, bool kin, hipStream_t s, const Work *w = nullptr) {
     const size_t cs = sizeof(cplx<R>);
     const int nanl = pv_.nanl;
     const int rows = pv_.rows;
@@ -606,6 +694,9 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
     if (sizeof(R) == 8) {
         anl = (const cplx<R> *)pv_.anl64->p;
     } else {
+        if (!*pv_.anl32_valid && w)
+            return fail(CPWGPU_ERR_ARG, "no FP32 copy of the projectors: it is made by the first FP32 apply "
+                                        "(cpwgpu_plan_apply) after the projectors are set");
         if (!*pv_.anl32_valid) {
             *pv_.lda32 = (pv_.lda + 1) & ~1L;
             if ((st = pv_.anl32->reserve(sizeof(cplx<float>) * (size_t)*pv_.lda32 * nanl))) return st;
@@ -620,30 +711,30 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
     }
     const long lda = sizeof(R) == 8 ? pv_.lda : *pv_.lda32;
     const int ix = sizeof(R) == 4 ? 1 : 0;
-    const int pv = p->proj_var[ix] >= 1 && p->proj_var[ix] < NPROJ_VAR ? p->proj_var[ix] : 0;
+    const int pv = proj_variant<R>(p);
     // block tile and K granule of the chosen projection kernel
-    int BM = proj_bm<R>(), BN = proj_bn<R>(), BK = GemmCfg<R>::BK;
-    if (pv == NPROJ_DIRECT + 1) {
-        BM = 2 * proj_half_wm<R>() * 16;
-        BN = 2 * proj_half_wn<R>() * 16;
-    }
-#define CPW_PD(V) if (pv == V) { BM = direct_bm<R, V>(); BN = direct_bn<R, V>(); BK = direct_bk<R, V>(); }
-    CPW_PD(1) CPW_PD(2) CPW_PD(3) CPW_PD(4)
-#undef CPW_PD
+    int BM, BN, BK;
+    proj_tile<R>(pv, &BM, &BN, &BK);
     const int tm = (nanl + BM - 1) / BM, tn = (neig + BN - 1) / BN;
-    // K slices are whole LDS stages of k_proj / whole chunks of k_proj_direct
-    const int ksteps = (rows + BK - 1) / BK;
-    const int target = p->splitk[ix] > 0 ? p->splitk[ix] : 256;
-    int nsl = std::max(1, std::min(ksteps, (target + tm * tn - 1) / (tm * tn)));
-    const int kslice = ((ksteps + nsl - 1) / nsl) * BK;
-    nsl = (rows + kslice - 1) / kslice;
-    if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
-    if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
+    int nsl, kslice;
+    splitk_shape<R>(p, rows, nanl, neig, &nsl, &kslice);
+    cplx<R> *part, *dhd;
+    if (w) {
+        if (cs * (size_t)nsl * neig * nanl > w->part_n || cs * (size_t)neig * nanl > w->dhd_n)
+            return fail(CPWGPU_ERR_SIZE, "non-local workspace too small");
+        part = (cplx<R> *)w->part;
+        dhd = (cplx<R> *)w->dhd;
+    } else {
+        if ((st = p->part.reserve(cs * (size_t)nsl * neig * nanl))) return st;
+        if ((st = p->dhd.reserve(cs * (size_t)neig * nanl))) return st;
+        part = (cplx<R> *)p->part.p;
+        dhd = (cplx<R> *)p->dhd.p;
+    }
     {
         ProfScope ps_(CPWGPU_K_PROJ, s);
         const dim3 grid(tm, tn, nsl);
         const double2 *ps = (const double2 *)psi;
-        cplx<R> *pt = (cplx<R> *)p->part.p;
+        cplx<R> *pt = part;
         if (pv == 1)
             hipLaunchKernelGGL((k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
         else if (pv == 2)
@@ -655,21 +746,19 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
         else if (pv == NPROJ_DIRECT + 1)
             hipLaunchKernelGGL((k_proj<R, proj_half_wm<R>(), proj_half_wn<R>()>), dim3(tm, tn, nsl),
                                dim3(GEMM_THREADS), 0, s, anl, lda, (const double2 *)psi, ldp,
-                               (cplx<R> *)p->part.p, rows, nanl, neig, kslice, nullptr);
+                               pt, rows, nanl, neig, kslice, nullptr);
         else
             hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const double2 *)psi, ldp, (cplx<R> *)p->part.p, rows, nanl, neig,
-                               kslice, nullptr);
+                               (const double2 *)psi, ldp, pt, rows, nanl, neig, kslice, nullptr);
     }
     {
         ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
-        launch_proj_reduce<R>((const cplx<R> *)p->part.p, pv_.xnl,
-                              (cplx<R> *)p->dhd.p, nanl, neig, nsl, s);
+        launch_proj_reduce<R>((const cplx<R> *)part, pv_.xnl, dhd, nanl, neig, nsl, s);
     }
     {
         ProfScope ps_(CPWGPU_K_BACK, s);
         const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_VAR - 1 ? p->back_var[ix] : 0;
-        const cplx<R> *dh = (const cplx<R> *)p->dhd.p;
+        const cplx<R> *dh = dhd;
         double2 *hp = (double2 *)hpsi;
         // the kinetic term ekpg * psi is added here (coalesced), not in the gather
         const double2 *kp = kin ? (const double2 *)psi : nullptr;
@@ -716,16 +805,23 @@ int ensure_aux() {
 // hpsi; join (may be null): event the first gather waits for (that term).
 template <typename R>
 int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-               bool acc, hipEvent_t join, hipStream_t s, bool kin_gather = false) {
+               bool acc, hipEvent_t join, hipStream_t s, bool kin_gather = false, const Work *w = nullptr) {
     const FftGeom &g = p->g;
     const size_t cs = sizeof(cplx<R>);
     const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
     if (!vr) return fail(CPWGPU_ERR_ARG, "potential not set");
     const size_t per_band = cs * (size_t)g.n[2] * g.ncolp;
-    const int nbc = (int)std::max<size_t>(1, std::min<size_t>(neig, (4ull << 30) / per_band));
-    int st = p->colbuf.reserve(per_band * nbc);
-    if (st) return st;
-    cplx<R> *colbuf = (cplx<R> *)p->colbuf.p;
+    const size_t cap = w ? w->colbuf_n : (4ull << 30);
+    if (per_band > cap) return fail(CPWGPU_ERR_SIZE, "z-column workspace too small for one band");
+    const int nbc = (int)std::max<size_t>(1, std::min<size_t>(neig, cap / per_band));
+    int st;
+    cplx<R> *colbuf;
+    if (w) {
+        colbuf = (cplx<R> *)w->colbuf;
+    } else {
+        if ((st = p->colbuf.reserve(per_band * nbc))) return st;
+        colbuf = (cplx<R> *)p->colbuf.p;
+    }
     bool joined = join == nullptr;
     for (int b0 = 0; b0 < neig; b0 += nbc) {
         const int nb = std::min(nbc, neig - b0);
@@ -748,28 +844,30 @@ int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
     return CPWGPU_OK;
 }
 
+// w: caller workspace (see Work), or null for the plan's buffers.  With a
+// workspace the second stream is used only when it already exists.
 template <typename R>
 int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-            hipStream_t s) {
+            hipStream_t s, const Work *w = nullptr) {
     const bool nl = p->nanl > 0 && p->g.mtxd > 0;
     // kinetic term in the gather (kg) or in the back-projection epilogue
     const bool kg = p->kin_var[sizeof(R) == 4 ? 1 : 0] == 1;
     int st;
-    if (nl && !g_overlap) {
+    if (nl && (!g_overlap || (w && !g_aux))) {
         // serial: non-local term first (writes hpsi), then the local term adds
-        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, s))) return st;
-        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s, kg);
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, s, w))) return st;
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s, kg, w);
     }
     if (nl) {
         // fork: non-local term on the second stream, written into hpsi
         if ((st = ensure_aux())) return st;
         HIPCHK(hipEventRecord(g_fork, s));
         HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
-        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, g_aux))) return st;
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, g_aux, w))) return st;
         HIPCHK(hipEventRecord(g_join, g_aux));
-        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s, kg);
+        return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, g_join, s, kg, w);
     }
-    return local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s);
+    return local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s, false, w);
 }
 
 int set_potential(cpwgpu_plan_s *p, const double *vscr, int mxdscr) {
@@ -798,22 +896,57 @@ int set_potential(cpwgpu_plan_s *p, const double *vscr, int mxdscr) {
     return CPWGPU_OK;
 }
 
+// Replaces the plan's projectors with nanl columns: fill(anl, lda) writes
+// them into the FP64 buffer (reserved for lda = max(1, mtxd) rows) and sets
+// xnl, enqueued on g_stream.  nanl, lda and the FP32 copy change only after
+// every step succeeded; a failure after the old buffer was released leaves
+// the plan without projectors (nanl = 0), never with a partly written set.
+// An FP32 copy that exists is converted again at once (same buffer when it
+// is large enough), so a captured graph that reads it sees the new values.
+template <typename F> int replace_projectors(cpwgpu_plan_s *p, int nanl, F fill) {
+    int st = ensure_stream();
+    if (st) return st;
+    if (nanl == 0) {
+        p->nanl = 0;
+        p->anl32_valid = false;
+        return CPWGPU_OK;
+    }
+    const long lda = std::max(1, p->mtxd);
+    st = p->anl64.reserve(16 * (size_t)lda * nanl);
+    if (!st) st = p->xnl.reserve(8 * (size_t)nanl);
+    if (!st) st = fill((double2 *)p->anl64.p, lda);
+    if (st) {
+        p->nanl = 0;
+        p->anl32_valid = false;
+        return st;
+    }
+    p->nanl = nanl;
+    p->lda = lda;
+    p->anl32_valid = false;
+    const long lda32 = (lda + 1) & ~1L;
+    if (p->anl32.p && p->anl32.n >= sizeof(cplx<float>) * (size_t)lda32 * nanl) {
+        p->lda32 = lda32;
+        const long tot = (long)p->mtxd * nanl;
+        if (tot > 0) {
+            hipLaunchKernelGGL(k_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, g_stream,
+                               (const double2 *)p->anl64.p, lda, (cplx<float> *)p->anl32.p, lda32, p->mtxd, nanl);
+            HIPCHK(hipGetLastError());
+        }
+        p->anl32_valid = true;
+    }
+    return CPWGPU_OK;
+}
+
 int set_projectors_host(cpwgpu_plan_s *p, const double *anlga, const double *xnlkb, int nanl,
                         int mxddim) {
     if (nanl < 0) return fail(CPWGPU_ERR_ARG, "nanl = %d", nanl);
-    int st = ensure_stream();
-    if (st) return st;
-    p->nanl = nanl;
-    p->anl32_valid = false;
-    if (nanl == 0) return CPWGPU_OK;
-    if (!anlga || !xnlkb) return fail(CPWGPU_ERR_ARG, "null projector array");
-    p->lda = std::max(1, p->mtxd);
-    if ((st = p->anl64.reserve(16 * (size_t)p->lda * nanl))) return st;
-    if (p->mtxd > 0)
-        HIPCHK(hipMemcpy2DAsync(p->anl64.p, 16 * (size_t)p->lda, anlga, 16 * (size_t)mxddim,
-                                16 * (size_t)p->mtxd, nanl, hipMemcpyHostToDevice, g_stream));
-    if ((st = upload(p->xnl, xnlkb, nanl))) return st;
-    return CPWGPU_OK;
+    if (nanl > 0 && (!anlga || !xnlkb)) return fail(CPWGPU_ERR_ARG, "null projector array");
+    return replace_projectors(p, nanl, [&](double2 *anl, long lda) -> int {
+        if (p->mtxd > 0)
+            HIPCHK(hipMemcpy2DAsync(anl, 16 * (size_t)lda, anlga, 16 * (size_t)mxddim, 16 * (size_t)p->mtxd, nanl,
+                                    hipMemcpyHostToDevice, g_stream));
+        return upload(p->xnl, xnlkb, nanl);
+    });
 }
 
 // ---- KB projectors on the device (Src/omp/proj_nl_kb_c16.f90:23-275) --------------
@@ -968,18 +1101,26 @@ int set_potential_spin(cpwgpu_plan_s *p, const double *vscr_sp, int mxdscr, int 
 int set_projectors_spin_host(cpwgpu_plan_s *p, const double *anlsp, const double *xnlkbsp, int nanlsp,
                              int mxddim) {
     if (nanlsp < 0) return fail(CPWGPU_ERR_ARG, "nanlsp = %d", nanlsp);
+    if (nanlsp > 0 && (!anlsp || !xnlkbsp)) return fail(CPWGPU_ERR_ARG, "null projector array");
     int st = ensure_stream();
     if (st) return st;
-    p->snanl = nanlsp;
+    // as replace_projectors: the count changes only once the new set is in place
     p->sanl32_valid = false;
-    if (nanlsp == 0) return CPWGPU_OK;
-    if (!anlsp || !xnlkbsp) return fail(CPWGPU_ERR_ARG, "null projector array");
-    p->slda = std::max(1, 2 * p->mtxd);
-    if ((st = p->sanl64.reserve(16 * (size_t)p->slda * nanlsp))) return st;
-    if (p->mtxd > 0)
-        HIPCHK(hipMemcpy2DAsync(p->sanl64.p, 16 * (size_t)p->slda, anlsp, 16 * (size_t)(2 * (size_t)mxddim),
-                                16 * (size_t)(2 * p->mtxd), nanlsp, hipMemcpyHostToDevice, g_stream));
-    return upload(p->sxnl, xnlkbsp, nanlsp);
+    if (nanlsp == 0) {
+        p->snanl = 0;
+        return CPWGPU_OK;
+    }
+    const long slda = std::max(1, 2 * p->mtxd);
+    st = p->sanl64.reserve(16 * (size_t)slda * nanlsp);
+    if (!st && p->mtxd > 0) {
+        const hipError_t e = hipMemcpy2DAsync(p->sanl64.p, 16 * (size_t)slda, anlsp, 16 * (size_t)(2 * (size_t)mxddim),
+                                              16 * (size_t)(2 * p->mtxd), nanlsp, hipMemcpyHostToDevice, g_stream);
+        if (e != hipSuccess) st = fail(CPWGPU_ERR_HIP, "projector copy: %s", hipGetErrorString(e));
+    }
+    if (!st) st = upload(p->sxnl, xnlkbsp, nanlsp);
+    p->snanl = st ? 0 : nanlsp;
+    if (!st) p->slda = slda;
+    return st;
 }
 
 // register shape of the coupled two-component plane kernel: (x lanes, y
@@ -2224,6 +2365,7 @@ int cpwgpu_init(int device) {
         g_stream = nullptr;
     }
     g_device = device;
+    HIPCHK(hipSetDevice(device));
     return ensure_stream();
 }
 
@@ -2328,19 +2470,15 @@ int cpwgpu_plan_set_projectors_device(cpwgpu_plan_t plan, const double *anlga_de
                                       const double *xnlkb_dev, int nanl) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!plan || nanl < 0) return fail(CPWGPU_ERR_ARG, "bad argument");
-    int st = ensure_stream();
+    if (nanl > 0 && (!anlga_dev || !xnlkb_dev || lda < plan->mtxd)) return fail(CPWGPU_ERR_ARG, "bad projector arrays");
+    int st = replace_projectors(plan, nanl, [&](double2 *anl, long ld) -> int {
+        if (plan->mtxd > 0)
+            HIPCHK(hipMemcpy2DAsync(anl, 16 * (size_t)ld, anlga_dev, 16 * (size_t)lda, 16 * (size_t)plan->mtxd, nanl,
+                                    hipMemcpyDeviceToDevice, g_stream));
+        HIPCHK(hipMemcpyAsync(plan->xnl.p, xnlkb_dev, 8 * (size_t)nanl, hipMemcpyDeviceToDevice, g_stream));
+        return CPWGPU_OK;
+    });
     if (st) return st;
-    plan->nanl = nanl;
-    plan->anl32_valid = false;
-    if (nanl == 0) return CPWGPU_OK;
-    if (!anlga_dev || !xnlkb_dev || lda < plan->mtxd) return fail(CPWGPU_ERR_ARG, "bad projector arrays");
-    plan->lda = std::max(1, plan->mtxd);
-    if ((st = plan->anl64.reserve(16 * (size_t)plan->lda * nanl))) return st;
-    if ((st = plan->xnl.reserve(8 * (size_t)nanl))) return st;
-    if (plan->mtxd > 0)
-        HIPCHK(hipMemcpy2DAsync(plan->anl64.p, 16 * (size_t)plan->lda, anlga_dev, 16 * (size_t)lda,
-                                16 * (size_t)plan->mtxd, nanl, hipMemcpyDeviceToDevice, g_stream));
-    HIPCHK(hipMemcpyAsync(plan->xnl.p, xnlkb_dev, 8 * (size_t)nanl, hipMemcpyDeviceToDevice, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
 }
@@ -2353,14 +2491,12 @@ int cpwgpu_plan_build_projectors_kb(cpwgpu_plan_t plan, const double *rkpt, int 
     if (!plan || nanl < 0) return fail(CPWGPU_ERR_ARG, "bad argument");
     KbSetup s;
     int st = kb_setup(rkpt, nanl, nqnl, delqnl, vkb, nkb, ntype, natom, rat, adot, mxdtyp, mxdatm, mxdlqp, s);
-    if (st || (st = ensure_stream())) return st;
-    plan->nanl = nanl;
-    plan->anl32_valid = false;
-    if (nanl == 0) return CPWGPU_OK;
-    plan->lda = std::max(1, plan->mtxd);
-    if ((st = plan->anl64.reserve(16 * (size_t)plan->lda * nanl)) || (st = upload(plan->xnl, s.xnl.data(), nanl)) ||
-        (st = kb_build(s, (const double *)plan->gw.p, plan->mtxd, (double2 *)plan->anl64.p, plan->lda)))
-        return st;
+    if (st) return st;
+    st = replace_projectors(plan, nanl, [&](double2 *anl, long lda) -> int {
+        int rc = upload(plan->xnl, s.xnl.data(), nanl);
+        return rc ? rc : kb_build(s, (const double *)plan->gw.p, plan->mtxd, anl, lda);
+    });
+    if (st) return st;
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
 }
@@ -2456,9 +2592,45 @@ int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long
 
 }  // extern "C"
 
+namespace {
+// What a captured apply has baked into the graph for one plan: the device
+// buffers it reads and writes and the projector shape.  Rebuilding the
+// projectors with the same count keeps every buffer (the FP32 copy is
+// converted again in place), so the graph stays valid; another count, or a
+// workspace reallocated by a larger apply, changes this record.
+struct PlanState {
+    const cpwgpu_plan_s *p = nullptr;
+    int nanl = 0;
+    long lda = 0, lda32 = 0;
+    bool anl32_valid = false;
+    const void *buf[9] = {};
+    bool operator==(const PlanState &o) const {
+        if (p != o.p || nanl != o.nanl || lda != o.lda || lda32 != o.lda32 || anl32_valid != o.anl32_valid)
+            return false;
+        for (int i = 0; i < 9; i++)
+            if (buf[i] != o.buf[i]) return false;
+        return true;
+    }
+};
+
+PlanState plan_state(const cpwgpu_plan_s *p) {
+    PlanState s;
+    s.p = p;
+    s.nanl = p->nanl;
+    s.lda = p->lda;
+    s.lda32 = p->lda32;
+    s.anl32_valid = p->anl32_valid;
+    const DevBuf *b[9] = {&p->anl64, &p->anl32, &p->xnl, &p->colbuf, &p->part, &p->dhd, &p->v64, &p->v32, &p->ekpg};
+    for (int i = 0; i < 9; i++) s.buf[i] = b[i]->p;
+    return s;
+}
+}  // namespace
+
 struct cpwgpu_graph_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int prec = CPWGPU_PREC_FP64;
+    std::vector<PlanState> state;  // per k-point, at capture
 };
 
 extern "C" {
@@ -2511,12 +2683,30 @@ int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plan
         if (st) return st;
         return fail(CPWGPU_ERR_HIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
     }
+    gr->prec = prec;
+    for (int k = 0; k < nk; k++) gr->state.push_back(plan_state(plans[k]));
     *graph = gr;
     return CPWGPU_OK;
 }
 
 int cpwgpu_graph_launch(cpwgpu_graph_t graph, void *stream) {
     if (!graph || !graph->exec) return fail(CPWGPU_ERR_ARG, "null graph");
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (size_t k = 0; k < graph->state.size(); k++) {
+            const PlanState &s = graph->state[k];
+            const PlanState now = plan_state(s.p);
+            // an FP64 graph does not read the FP32 projector copy
+            const bool same = graph->prec == CPWGPU_PREC_FP32
+                                  ? now == s
+                                  : (now.nanl == s.nanl && now.lda == s.lda && now.buf[0] == s.buf[0] &&
+                                     std::equal(now.buf + 2, now.buf + 9, s.buf + 2));
+            if (!same)
+                return fail(CPWGPU_ERR_ARG,
+                            "k-point %zu: the plan's projectors or buffers changed since the graph was "
+                            "captured (projector count, or a larger apply); create the graph again", k);
+        }
+    }
     HIPCHK(hipGraphLaunch(graph->exec, (hipStream_t)stream));
     return CPWGPU_OK;
 }
@@ -2869,17 +3059,15 @@ int cpwgpu_hk_psi_c8(const int *mtxd, const int *neig, const float *psi, float *
     // projectors: complex64 -> the plan's complex128 copy (the FP32 path rounds them back exactly);
     // host staging vectors live until the final synchronisation
     std::vector<double> x(xnlkb ? xnlkb : (const float *)nullptr, xnlkb ? xnlkb + na : (const float *)nullptr);
-    p->nanl = na;
-    p->anl32_valid = false;
-    if (na > 0) {
-        if (!anlga || !xnlkb) return fail(CPWGPU_ERR_ARG, "null projector array");
-        p->lda = m;
-        if ((st = p->anl64.reserve(c16b * na))) return st;
+    if (na > 0 && (!anlga || !xnlkb)) return fail(CPWGPU_ERR_ARG, "null projector array");
+    st = replace_projectors(p, na, [&](double2 *anl, long lda) -> int {
         HIPCHK(hipMemcpy2DAsync(g_drop.c8.p, c8b, anlga, 8 * (size_t)*mxddim, c8b, na, hipMemcpyHostToDevice, g_stream));
         hipLaunchKernelGGL(k_to_c128, grid((long)m * na), dim3(256), 0, g_stream, (const cplx<float> *)g_drop.c8.p,
-                           (long)m, (double2 *)p->anl64.p, (long)m, m, na);
-        if ((st = upload(p->xnl, x.data(), na))) return st;
-    }
+                           (long)m, anl, lda, m, na);
+        HIPCHK(hipGetLastError());
+        return upload(p->xnl, x.data(), na);
+    });
+    if (st) return st;
     HIPCHK(hipMemcpy2DAsync(g_drop.c8.p, c8b, psi, 8 * (size_t)*mxddim, c8b, n, hipMemcpyHostToDevice, g_stream));
     hipLaunchKernelGGL(k_to_c128, grid((long)m * n), dim3(256), 0, g_stream, (const cplx<float> *)g_drop.c8.p,
                        (long)m, (double2 *)g_drop.psi.p, (long)m, m, n);

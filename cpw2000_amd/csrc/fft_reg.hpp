@@ -19,6 +19,7 @@
 #pragma once
 
 #include "fft_fast.hpp"
+#include "fft_wave.hpp"
 
 namespace cpwgpu {
 
@@ -145,8 +146,10 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 // column, S = nx + 2) and the twiddle table twN[N].  Phases (barrier between each):
 //   gather columns -> inverse y (GY-lane lines over u) -> per row y: inverse
 //   x, V(r), forward x (GX-lane lines, no barrier) -> forward y -> scatter.
+// YW: the y lines instead go through the LDS two-pass transform in place in
+// sub, each line exchanged within one wavefront (line_fft_wave; GY unused).
 // ---------------------------------------------------------------------------
-template <typename R, int N, int GX, int GY, int P, int T, int W>
+template <typename R, int N, int GX, int GY, int P, int T, int W, bool YW = false>
 __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf,
                                                  const int *__restrict__ col_uy,
                                                  const R *__restrict__ vr,
@@ -177,7 +180,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     }
     __syncthreads();
 
-    {  // inverse y on the P * nx box lines
+    if constexpr (YW) {
+        for (int p = 0; p < P; p++) line_fft_wave<R, N, +1, T>(sub + p * N * S, nx, 1, S, twN);
+    } else {  // inverse y on the P * nx box lines
         const Lanes<R, GY> L;
         const int ntask = P * nx;
         for (int task = tid / GY; task < ntask; task += T / GY) {
@@ -221,7 +226,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     }
     __syncthreads();
 
-    {  // forward y
+    if constexpr (YW) {
+        for (int p = 0; p < P; p++) line_fft_wave<R, N, -1, T>(sub + p * N * S, nx, 1, S, twN);
+    } else {  // forward y
         const Lanes<R, GY> L;
         const int ntask = P * nx;
         for (int task = tid / GY; task < ntask; task += T / GY) {

@@ -30,47 +30,47 @@ subroutine ditsp_c16(ipr, ifail, icmax, iguess, epspsi, lnewanl,         &
 
 ! input
 
-  integer, intent(in)                ::  mxddim                          !<  array dimension of plane-waves
-  integer, intent(in)                ::  mxdsml                          !<  array dimension of small hamiltonian
-  integer, intent(in)                ::  mxdbnd                          !<  array dimension for number of bands
-  integer, intent(in)                ::  mxdgve                          !<  array dimension for g-space vectors
-  integer, intent(in)                ::  mxdscr                          !<  array dimension for fft transform
-  integer, intent(in)                ::  mxdanl                          !<  array dimension for number of KB projectors
+  integer, intent(in)                ::  mxddim                           ! leading dimension of the plane-wave arrays
+  integer, intent(in)                ::  mxdsml                           ! leading dimension of the small matrix
+  integer, intent(in)                ::  mxdbnd                           ! bound of the band count
+  integer, intent(in)                ::  mxdgve                           ! bound of the G list
+  integer, intent(in)                ::  mxdscr                           ! length of the potential array
+  integer, intent(in)                ::  mxdanl                           ! leading bound of the projector count
 
-  integer, intent(in)                ::  ipr                             !<  print control
-  integer, intent(in)                ::  icmax                           !<  maximum number of outer iterations
-  integer, intent(in)                ::  iguess                          !<  1: psi holds neig guess eigenvectors
-  real(REAL64), intent(in)           ::  epspsi                          !<  convergence criterion
+  integer, intent(in)                ::  ipr                              ! amount of printing
+  integer, intent(in)                ::  icmax                            ! iteration limit
+  integer, intent(in)                ::  iguess                           ! 1 when psi holds starting vectors
+  real(REAL64), intent(in)           ::  epspsi                           ! target |H psi - e psi| of every band
 
-  integer, intent(in)                ::  mtxd                            !<  dimension of the hamiltonian
-  integer, intent(in)                ::  mtxds                           !<  dimension of the small hamiltonian
+  integer, intent(in)                ::  mtxd                             ! number of plane waves (rows used)
+  integer, intent(in)                ::  mtxds                            ! order of the small starting matrix
 
-  real(REAL64), intent(in)           ::  ekpg(mxddim)                    !<  kinetic energy (hartree) of k+g-vector i
-  integer, intent(in)                ::  isort(mxddim)                   !<  g-vector of row/column i of the hamiltonian
-  real(REAL64), intent(in)           ::  vscr(mxdscr)                    !<  screened potential on the fft mesh
-  integer, intent(in)                ::  kmscr(7)                        !<  fft mesh limits and sizes
+  real(REAL64), intent(in)           ::  ekpg(mxddim)                     ! |k+G_i|^2 / 2 (Ha) of row i
+  integer, intent(in)                ::  isort(mxddim)                    ! index into kgv of the G of row i
+  real(REAL64), intent(in)           ::  vscr(mxdscr)                     ! V(r) on the FFT mesh, index (k3*n2+k2)*id+k1
+  integer, intent(in)                ::  kmscr(7)  ! (1:3) potential box, (4:6) mesh n1 n2 n3, (7) leading dim
 
-  integer, intent(in)                ::  ng                              !<  number of g-vectors
-  integer, intent(in)                ::  kgv(3,mxdgve)                   !<  g-vectors in reciprocal lattice coordinates
+  integer, intent(in)                ::  ng                               ! length of the G list
+  integer, intent(in)                ::  kgv(3,mxdgve)                    ! integer components of each G (reciprocal-lattice units)
 
-  complex(REAL64), intent(in)        ::  anlga(mxddim,mxdanl)            !<  KB projectors
-  real(REAL64), intent(in)           ::  xnlkb(mxdanl)                   !<  KB projector normalization
-  integer, intent(in)                ::  nanl                            !<  number of projectors
+  complex(REAL64), intent(in)        ::  anlga(mxddim,mxdanl)             ! projector beta_a(k+G_i): row i, column a
+  real(REAL64), intent(in)           ::  xnlkb(mxdanl)                    ! KB strength of each projector
+  integer, intent(in)                ::  nanl                             ! number of projectors in use
 
-  complex(REAL64), intent(in)        ::  hamsm(mxdsml,mxdsml)            !<  small hamiltonian
-  real(REAL64), intent(in)           ::  hdiag(mxddim)                   !<  hamiltonian diagonal
+  complex(REAL64), intent(in)        ::  hamsm(mxdsml,mxdsml)             ! leading mtxds x mtxds block of H
+  real(REAL64), intent(in)           ::  hdiag(mxddim)                    ! diagonal of H (preconditioner)
 
 ! output
 
-  integer, intent(out)               ::  ifail                           !<  0 on success, else the number of correct digits
-  real(REAL64), intent(out)          ::  ei(mxdbnd)                      !<  eigenvalues
+  integer, intent(out)               ::  ifail  ! 0 when converged, otherwise a measure of the residual reached
+  real(REAL64), intent(out)          ::  ei(mxdbnd)                       ! eigenvalues (Ha), ascending
 
 ! input and output
 
-  complex(REAL64), intent(inout)     ::  psi(mxddim,mxdbnd)              !<  |psi>
-  complex(REAL64), intent(inout)     ::  hpsi(mxddim,mxdbnd)             !<  H |psi>
-  integer, intent(inout)             ::  neig                            !<  number of eigenvectors (not changed here)
-  logical, intent(inout)             ::  lnewanl                         !<  read only here: some callers pass a literal
+  complex(REAL64), intent(inout)     ::  psi(mxddim,mxdbnd)               ! bands, one per column
+  complex(REAL64), intent(inout)     ::  hpsi(mxddim,mxdbnd)              ! H applied to psi, rows 1..mtxd
+  integer, intent(inout)             ::  neig                             ! number of bands
+  logical, intent(inout)             ::  lnewanl                          ! projector-change flag: read, never written
 
   interface
     integer(c_int) function cpwgpu_ditsp_c16(ipr, ifail, icmax, iguess, epspsi, lnewanl,       &

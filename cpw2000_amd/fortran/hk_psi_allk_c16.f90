@@ -27,33 +27,33 @@ subroutine hk_psi_allk_c16(nrk, mtxd_allk, nband, psi_allk, hpsi_allk,   &
 
 ! input
 
-  integer, intent(in)                ::  mxddim                          !<  array dimension of plane-waves
-  integer, intent(in)                ::  mxdbnd                          !<  array dimension for number of bands
-  integer, intent(in)                ::  mxdanl                          !<  array dimension of number of projectors
-  integer, intent(in)                ::  mxdgve                          !<  array dimension for g-space vectors
-  integer, intent(in)                ::  mxdscr                          !<  array dimension of vscr
-  integer, intent(in)                ::  mxdnrk                          !<  array dimension of k-points
+  integer, intent(in)                ::  mxddim                           ! leading dimension of the plane-wave arrays
+  integer, intent(in)                ::  mxdbnd                           ! bound of the band count
+  integer, intent(in)                ::  mxdanl                           ! leading bound of the projector count
+  integer, intent(in)                ::  mxdgve                           ! bound of the G list
+  integer, intent(in)                ::  mxdscr                           ! length of the potential array
+  integer, intent(in)                ::  mxdnrk                           ! bound of the k-point count
 
-  integer, intent(in)                ::  nrk                             !<  number of k-points
-  integer, intent(in)                ::  mtxd_allk(mxdnrk)               !<  wavefunction dimension of each k-point
-  integer, intent(in)                ::  nband(mxdnrk)                   !<  number of wavefunctions of each k-point
-  real(REAL64), intent(in)           ::  ekpg_allk(mxddim,mxdnrk)        !<  kinetic energy (hartree) of k+g-vector i of k-point n
-  integer, intent(in)                ::  isort_allk(mxddim,mxdnrk)       !<  g-vector of row/column i of the hamiltonian of k-point n
+  integer, intent(in)                ::  nrk                              ! number of k-points
+  integer, intent(in)                ::  mtxd_allk(mxdnrk)                ! number of plane waves of each k-point
+  integer, intent(in)                ::  nband(mxdnrk)                    ! bands of each k-point
+  real(REAL64), intent(in)           ::  ekpg_allk(mxddim,mxdnrk)         ! |k+G_i|^2 / 2 (Ha) of row i of each k-point
+  integer, intent(in)                ::  isort_allk(mxddim,mxdnrk)        ! index into kgv of the G of row i of each k-point
 
-  real(REAL64), intent(in)           ::  vscr(mxdscr)                    !<  screened potential in the fft real space mesh
-  integer, intent(in)                ::  kmscr(7)                        !<  max value of kgv(i,n) used for the potential fft mesh and fft mesh size
+  real(REAL64), intent(in)           ::  vscr(mxdscr)                     ! V(r) on the FFT mesh, index (k3*n2+k2)*id+k1
+  integer, intent(in)                ::  kmscr(7)  ! (1:3) potential box, (4:6) mesh n1 n2 n3, (7) leading dim
 
-  integer, intent(in)                ::  ng                              !<  total number of g-vectors with length less than gmax
-  integer, intent(in)                ::  kgv(3,mxdgve)                   !<  G-vectors in reciprocal lattice coordinates
+  integer, intent(in)                ::  ng                               ! length of the G list
+  integer, intent(in)                ::  kgv(3,mxdgve)                    ! integer components of each G (reciprocal-lattice units)
 
-  integer, intent(in)                ::  nanl_allk(mxdnrk)               !<  number of projectors of each k-point
-  complex(REAL64), intent(in)        ::  anlga_allk(mxddim,mxdanl,mxdnrk) !<  Kleinman-Bylander projectors of each k-point
-  real(REAL64), intent(in)           ::  xnlkb_allk(mxdanl,mxdnrk)       !<  Kleinman-Bylander normalization of each k-point
-  complex(REAL64), intent(in)        ::  psi_allk(mxddim,mxdbnd,mxdnrk)  !<  wavevectors of each k-point
+  integer, intent(in)                ::  nanl_allk(mxdnrk)                ! number of projectors of each k-point
+  complex(REAL64), intent(in)        ::  anlga_allk(mxddim,mxdanl,mxdnrk) ! projectors of each k-point, as anlga
+  real(REAL64), intent(in)           ::  xnlkb_allk(mxdanl,mxdnrk)        ! KB strengths of each k-point
+  complex(REAL64), intent(in)        ::  psi_allk(mxddim,mxdbnd,mxdnrk)   ! bands of each k-point
 
 ! output
 
-  complex(REAL64), intent(inout)     ::  hpsi_allk(mxddim,mxdbnd,mxdnrk) !<  H |psi> of each k-point (other entries kept)
+  complex(REAL64), intent(inout)     ::  hpsi_allk(mxddim,mxdbnd,mxdnrk)  ! H psi of each k-point (other entries kept)
 
   interface
     integer(c_int) function cpwgpu_hk_psi_allk_c16(nrk, mtxd_allk, nband, psi_allk, hpsi_allk, &

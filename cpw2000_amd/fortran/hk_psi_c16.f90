@@ -25,35 +25,35 @@ subroutine hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl,                    &
 
 ! input
 
-  integer, intent(in)                ::  mxddim                          !<  array dimension of plane-waves
-  integer, intent(in)                ::  mxdbnd                          !<  array dimension for number of bands
-  integer, intent(in)                ::  mxdanl                          !<  array dimension of number of projectors
-  integer, intent(in)                ::  mxdgve                          !<  array dimension for g-space vectors
-  integer, intent(in)                ::  mxdscr                          !<  array dimension of vscr
+  integer, intent(in)                ::  mxddim                           ! leading dimension of the plane-wave arrays
+  integer, intent(in)                ::  mxdbnd                           ! bound of the band count
+  integer, intent(in)                ::  mxdanl                           ! leading bound of the projector count
+  integer, intent(in)                ::  mxdgve                           ! bound of the G list
+  integer, intent(in)                ::  mxdscr                           ! length of the potential array
 
-  integer, intent(in)                ::  mtxd                            !<  wavefunction dimension (basis size)
-  integer, intent(in)                ::  neig                            !<  number of wavefunctions
-  real(REAL64), intent(in)           ::  ekpg(mxddim)                    !<  kinetic energy (hartree) of k+g-vector of row/column i
-  integer, intent(in)                ::  isort(mxddim)                   !<  g-vector associated with row/column i of hamiltonian
+  integer, intent(in)                ::  mtxd                             ! number of plane waves (rows used)
+  integer, intent(in)                ::  neig                             ! number of bands
+  real(REAL64), intent(in)           ::  ekpg(mxddim)                     ! |k+G_i|^2 / 2 (Ha) of row i
+  integer, intent(in)                ::  isort(mxddim)                    ! index into kgv of the G of row i
 
-  real(REAL64), intent(in)           ::  vscr(mxdscr)                    !<  screened potential in the fft real space mesh
-  integer, intent(in)                ::  kmscr(7)                        !<  max value of kgv(i,n) used for the potential fft mesh and fft mesh size
+  real(REAL64), intent(in)           ::  vscr(mxdscr)                     ! V(r) on the FFT mesh, index (k3*n2+k2)*id+k1
+  integer, intent(in)                ::  kmscr(7)  ! (1:3) potential box, (4:6) mesh n1 n2 n3, (7) leading dim
 
-  integer, intent(in)                ::  ng                              !<  total number of g-vectors with length less than gmax
-  integer, intent(in)                ::  kgv(3,mxdgve)                   !<  G-vectors in reciprocal lattice coordinates
+  integer, intent(in)                ::  ng                               ! length of the G list
+  integer, intent(in)                ::  kgv(3,mxdgve)                    ! integer components of each G (reciprocal-lattice units)
 
-  integer, intent(in)                ::  nanl                            !<  number of projectors
-  complex(REAL64), intent(in)        ::  anlga(mxddim,mxdanl)            !<  Kleinman-Bylander projectors
-  real(REAL64), intent(in)           ::  xnlkb(mxdanl)                   !<  Kleinman-Bylander normalization
-  complex(REAL64), intent(in)        ::  psi(mxddim,mxdbnd)              !<  wavevector
+  integer, intent(in)                ::  nanl                             ! number of projectors in use
+  complex(REAL64), intent(in)        ::  anlga(mxddim,mxdanl)             ! projector beta_a(k+G_i): row i, column a
+  real(REAL64), intent(in)           ::  xnlkb(mxdanl)                    ! KB strength of each projector
+  complex(REAL64), intent(in)        ::  psi(mxddim,mxdbnd)               ! bands, one per column
 
 ! input and output (read only here: some callers pass a literal)
 
-  logical, intent(inout)             ::  lnewanl                         !<  indicates that anlga has been recalculated
+  logical, intent(inout)             ::  lnewanl                          ! projector-change flag: read, never written
 
 ! output
 
-  complex(REAL64), intent(out)       ::  hpsi(mxddim,mxdbnd)             !<  |hpsi> = H |psi>
+  complex(REAL64), intent(out)       ::  hpsi(mxddim,mxdbnd)              ! H applied to psi, rows 1..mtxd
 
   interface
     integer(c_int) function cpwgpu_hk_psi_c16(mtxd, neig, psi, hpsi, lnewanl,   &

@@ -24,37 +24,37 @@ subroutine hk_psi_spin_c16(mtxd, neig, psi_sp, hpsi_sp, lnewanl,         &
 
 ! input
 
-  integer, intent(in)                ::  mxddim                          !<  array dimension of plane-waves
-  integer, intent(in)                ::  mxdbnd                          !<  array dimension for number of bands
-  integer, intent(in)                ::  mxdasp                          !<  array dimension of number of projectors
-  integer, intent(in)                ::  mxdgve                          !<  array dimension for g-space vectors
-  integer, intent(in)                ::  mxdscr                          !<  array dimension of vscr_sp
-  integer, intent(in)                ::  mxdnsp                          !<  array dimension for number of spin components
+  integer, intent(in)                ::  mxddim                           ! leading dimension of the plane-wave arrays
+  integer, intent(in)                ::  mxdbnd                           ! bound of the band count
+  integer, intent(in)                ::  mxdasp                           ! bound of the spinor projector count
+  integer, intent(in)                ::  mxdgve                           ! bound of the G list
+  integer, intent(in)                ::  mxdscr                           ! length of the potential array
+  integer, intent(in)                ::  mxdnsp                           ! bound of the potential components
 
-  integer, intent(in)                ::  mtxd                            !<  wavefunction dimension (without spin)
-  integer, intent(in)                ::  neig                            !<  number of wavefunctions
-  real(REAL64), intent(in)           ::  ekpg(mxddim)                    !<  kinetic energy (hartree) of k+g-vector i
-  integer, intent(in)                ::  isort(mxddim)                   !<  g-vector of row/column i
+  integer, intent(in)                ::  mtxd                             ! number of plane waves (rows used)
+  integer, intent(in)                ::  neig                             ! number of bands
+  real(REAL64), intent(in)           ::  ekpg(mxddim)                     ! |k+G_i|^2 / 2 (Ha) of row i
+  integer, intent(in)                ::  isort(mxddim)                    ! index into kgv of the G of row i
 
-  real(REAL64), intent(in)           ::  vscr_sp(mxdscr,mxdnsp)          !<  potential components (1, sigma_z, sigma_x, sigma_y)
-  integer, intent(in)                ::  kmscr(7)                        !<  fft mesh limits and sizes
-  integer, intent(in)                ::  nsp                             !<  number of potential components (1,2,4)
+  real(REAL64), intent(in)           ::  vscr_sp(mxdscr,mxdnsp)           ! V(r) components: scalar, sigma_z, sigma_x, sigma_y
+  integer, intent(in)                ::  kmscr(7)  ! (1:3) potential box, (4:6) mesh n1 n2 n3, (7) leading dim
+  integer, intent(in)                ::  nsp                              ! potential components in use: 1, 2 or 4
 
-  integer, intent(in)                ::  ng                              !<  number of g-vectors
-  integer, intent(in)                ::  kgv(3,mxdgve)                   !<  g-vectors in reciprocal lattice coordinates
+  integer, intent(in)                ::  ng                               ! length of the G list
+  integer, intent(in)                ::  kgv(3,mxdgve)                    ! integer components of each G (reciprocal-lattice units)
 
-  integer, intent(in)                ::  nanlsp                          !<  number of projectors
-  complex(REAL64), intent(in)        ::  anlsp(2*mxddim,mxdasp)          !<  KB projectors on the spinor rows
-  real(REAL64), intent(in)           ::  xnlkbsp(mxdasp)                 !<  KB normalization
-  complex(REAL64), intent(in)        ::  psi_sp(2*mxddim,mxdbnd)         !<  spinor wavefunctions
+  integer, intent(in)                ::  nanlsp                           ! number of spinor projectors in use
+  complex(REAL64), intent(in)        ::  anlsp(2*mxddim,mxdasp)           ! projectors on the interleaved up/down rows
+  real(REAL64), intent(in)           ::  xnlkbsp(mxdasp)                  ! KB strength of each spinor projector
+  complex(REAL64), intent(in)        ::  psi_sp(2*mxddim,mxdbnd)          ! two-component bands, up/down rows interleaved
 
 ! input and output (read only here)
 
-  logical, intent(inout)             ::  lnewanl                         !<  indicates that anlsp has been recalculated
+  logical, intent(inout)             ::  lnewanl                          ! projector-change flag: read, never written
 
 ! output
 
-  complex(REAL64), intent(out)       ::  hpsi_sp(2*mxddim,mxdbnd)        !<  H |psi_sp>
+  complex(REAL64), intent(out)       ::  hpsi_sp(2*mxddim,mxdbnd)         ! H applied to psi_sp, rows 1..2*mtxd
 
   interface
     integer(c_int) function cpwgpu_hk_psi_spin_c16(mtxd, neig, psi_sp, hpsi_sp, lnewanl,      &

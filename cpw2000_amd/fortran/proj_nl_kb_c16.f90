@@ -23,33 +23,33 @@ subroutine proj_nl_kb_c16(rkpt, mtxd, isort, nanl,                       &
 
 ! input
 
-  integer, intent(in)                ::  mxdtyp                          !<  array dimension of types of atoms
-  integer, intent(in)                ::  mxdatm                          !<  array dimension of number of atoms of a given type
-  integer, intent(in)                ::  mxddim                          !<  array dimension of plane-waves
-  integer, intent(in)                ::  mxdlqp                          !<  array dimension for local potential
-  integer, intent(in)                ::  mxdanl                          !<  array dimension of number of projectors
-  integer, intent(in)                ::  mxdgve                          !<  array dimension of G-space vectors
+  integer, intent(in)                ::  mxdtyp                           ! bound of the species count
+  integer, intent(in)                ::  mxdatm                           ! bound of atoms per species
+  integer, intent(in)                ::  mxddim                           ! leading dimension of the plane-wave arrays
+  integer, intent(in)                ::  mxdlqp                           ! last index of the vkb tables
+  integer, intent(in)                ::  mxdanl                           ! leading bound of the projector count
+  integer, intent(in)                ::  mxdgve                           ! bound of the G list
 
-  real(REAL64), intent(in)           ::  rkpt(3)                         !<  k-point reciprocal lattice coordinates
-  integer, intent(in)                ::  mtxd                            !<  wavefunction dimension
-  integer, intent(in)                ::  isort(mxddim)                   !<  G-vector corresponding to coefficient i of wavefunction
+  real(REAL64), intent(in)           ::  rkpt(3)                          ! k-point, reciprocal-lattice coordinates
+  integer, intent(in)                ::  mtxd                             ! number of plane waves (rows used)
+  integer, intent(in)                ::  isort(mxddim)                    ! index into kgv of the G of row i
 
-  integer, intent(in)                ::  nanl                            !<  number of projectors without spin
-  integer, intent(in)                ::  ng                              !<  size of g-space
-  integer, intent(in)                ::  kgv(3,mxdgve)                   !<  G-vectors in reciprocal lattice coordinates
-  integer, intent(in)                ::  ntype                           !<  number of types of atoms
-  integer, intent(in)                ::  natom(mxdtyp)                   !<  number of atoms of type i
-  real(REAL64), intent(in)           ::  rat(3,mxdatm,mxdtyp)            !<  lattice coordinates of atom j of type i
-  real(REAL64), intent(in)           ::  adot(3,3)                       !<  metric in real space
-  integer, intent(in)                ::  nqnl(mxdtyp)                    !<  number of points for pseudo interpolation for atom k
-  real(REAL64), intent(in)           ::  delqnl(mxdtyp)                  !<  step used in the pseudo interpolation for atom k
-  real(REAL64), intent(in)        ::  vkb(-2:mxdlqp,0:LMAX,-1:1,mxdtyp)  !<  (1/q**l) * KB nonlocal pseudo. for atom k, ang. mom. l. NOT normalized to vcell, hartree
-  integer, intent(in)                ::  nkb(0:LMAX,-1:1,mxdtyp)         !<  KB pseudo.  normalization for atom k, ang. mom. l
+  integer, intent(in)                ::  nanl                             ! number of projectors in use
+  integer, intent(in)                ::  ng                               ! length of the G list
+  integer, intent(in)                ::  kgv(3,mxdgve)                    ! integer components of each G (reciprocal-lattice units)
+  integer, intent(in)                ::  ntype                            ! number of species
+  integer, intent(in)                ::  natom(mxdtyp)                    ! atoms of each species
+  real(REAL64), intent(in)           ::  rat(3,mxdatm,mxdtyp)             ! atom positions, fractional (lattice) coordinates
+  real(REAL64), intent(in)           ::  adot(3,3)                        ! real-space metric a_i . a_j (bohr^2)
+  integer, intent(in)                ::  nqnl(mxdtyp)                     ! points of the vkb tables of each species
+  real(REAL64), intent(in)           ::  delqnl(mxdtyp)                   ! q step of the vkb tables of each species
+  real(REAL64), intent(in)        ::  vkb(-2:mxdlqp,0:LMAX,-1:1,mxdtyp)   ! radial KB tables over q (divided by q^l)
+  integer, intent(in)                ::  nkb(0:LMAX,-1:1,mxdtyp)          ! sign (+1/-1/0) of each (l, species) KB term
 
 ! output
 
-  complex(REAL64), intent(out)       ::  anlga(mxddim,mxdanl)            !<  KB projectors without spin-orbit
-  real(REAL64), intent(out)          ::  xnlkb(mxdanl)                   !<  KB normalization without spin-orbit
+  complex(REAL64), intent(out)       ::  anlga(mxddim,mxdanl)             ! projector beta_a(k+G_i): row i, column a
+  real(REAL64), intent(out)          ::  xnlkb(mxdanl)                    ! KB strength of each projector
 
   interface
     integer(c_int) function cpwgpu_proj_nl_kb_c16(rkpt, mtxd, isort, nanl, ng, kgv, nqnl, delqnl, vkb, nkb, &

@@ -56,6 +56,10 @@ EXPORTS = (
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
 TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN = 0, 1, 2, 3, 4, 5
+# plane-stage kernels (values of TUNE_PLANE): 0 LDS two-pass, 1-3 register
+# lines, 4-6 LDS two-pass exchanged within one wavefront, 7 register x rows
+# with wave-exchanged LDS y lines
+NPLANE_VAR = 8
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back", "k_proj_kb")

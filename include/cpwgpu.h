@@ -183,7 +183,12 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 /* Tunable choices of a plan, per precision (CPWGPU_PREC_*):
  *   CPWGPU_TUNE_PLANE   kernel of the y/x plane stage: 0 = LDS two-pass lines,
  *                       1..3 = register-resident lines (DPP butterflies) in
- *                       three shapes; -1 = the default for the mesh
+ *                       three shapes, 4..6 = LDS two-pass lines exchanged
+ *                       within one wavefront (256, 192, 128 threads per
+ *                       block; where the plane fits in LDS), 7 = register
+ *                       x rows of shape 1 with the y lines through LDS
+ *                       exchanged within one wavefront; -1 = the default for
+ *                       the mesh
  *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
  *                       GEMM (64..4096); -1 = default (256)
  *   CPWGPU_TUNE_PROJ    kernel of the projection GEMM: 0 = LDS-staged tiles,
@@ -258,12 +263,21 @@ typedef struct cpwgpu_graph_s *cpwgpu_graph_t;
  * HIP graph (each k-point's non-local GEMMs overlap its FFT chain, as in
  * cpwgpu_plan_apply).  The batch is applied once eagerly first (buffers,
  * kernel attributes), then captured; the device arrays must stay valid while
- * the graph exists.  Replaces the k-loop of
- * Src/cpw/cpw_scf.f90:488-544 around hk_psi_c16 for a batch of k-points. */
+ * the graph exists, and the graph must be destroyed before its plans.
+ * Replaces the k-loop of Src/cpw/cpw_scf.f90:488-544 around hk_psi_c16 for a
+ * batch of k-points.
+ * The graph reads the plans' potential and projectors where they live, so a
+ * replay sees new values set with the same sizes (cpwgpu_plan_set_potential,
+ * or projectors of the same count from any cpwgpu_plan_set_projectors* /
+ * cpwgpu_plan_build_projectors_kb call: the FP32 copy is converted again in
+ * place by those calls). */
 int cpwgpu_graph_create(cpwgpu_graph_t *graph, int nk, const cpwgpu_plan_t *plans, const int *neig,
                         const double *const *psi, const long *ldp, double *const *hpsi,
                         const long *ldh, int prec);
-/* One launch of the recorded graph on stream (NULL = null stream). */
+/* One launch of the recorded graph on stream (NULL = null stream).  Returns
+ * CPWGPU_ERR_ARG without launching when a plan changed in a way the recording
+ * cannot follow: another projector count, or a plan workspace reallocated by
+ * a larger apply since the capture; create the graph again then. */
 int cpwgpu_graph_launch(cpwgpu_graph_t graph, void *stream);
 int cpwgpu_graph_destroy(cpwgpu_graph_t graph);
 

@@ -446,3 +446,13 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* Sets the OpenMP thread count of later calls (n < 1: all host cores). */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n < 1 ? omp_get_num_procs() : n);
+#else
+    (void)n;
+#endif
+}

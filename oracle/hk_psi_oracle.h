@@ -25,6 +25,7 @@ int orc_hk_psi_c16(int mtxd, int neig, const double *psi, double *hpsi, int ng, 
                    const double *anlga, const double *xnlkb, int nanl, int mxddim, int mxdbnd,
                    int mxdanl, int mxdgve, int mxdscr);
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

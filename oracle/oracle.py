@@ -57,6 +57,8 @@ def orc_lib():
         _orc.orc_hk_psi_c16.restype = C.c_int
         _orc.orc_cfft_wf_c16.restype = C.c_int
         _orc.orc_num_threads.restype = C.c_int
+        _orc.orc_set_num_threads.restype = None
+        _orc.orc_set_num_threads.argtypes = [C.c_int]
     return _orc
 
 
@@ -189,3 +191,21 @@ def conv_apply(d, psi, vq_g=None, vq=None):
     a = d["anlga"][:mtxd, :nanl]
     out += a @ (d["xnlkb"][:nanl, None] * (a.conj().T @ psi))
     return out
+
+
+def ref_ditsp_c16(a):
+    """The reference ditsp_c16 (Src/ditsp/ditsp_c16.f90:20-27, flang build
+    with the vendored BLAS / LAPACK) on the argument dict ``a`` (the names of
+    the Fortran dummies; psi, hpsi, ei written in place).  Returns ifail."""
+    lib = ref_lib()
+    i = lambda v: C.byref(C.c_int(int(v)))
+    ifail = C.c_int(0)
+    lnew = C.c_int(1)
+    neig = C.c_int(int(a["neig"]))
+    eps = C.c_double(float(a["epspsi"]))
+    lib.ditsp_c16_(i(a["ipr"]), C.byref(ifail), i(a["icmax"]), i(a["iguess"]), C.byref(eps), C.byref(lnew),
+                   C.byref(neig), i(a["mtxd"]), i(a["mtxds"]), _ptr(a["psi"]), _ptr(a["hpsi"]), _ptr(a["ei"]),
+                   _ptr(a["ekpg"]), _ptr(a["isort"]), _ptr(a["vscr"]), _ptr(a["kmscr"]), i(a["ng"]), _ptr(a["kgv"]),
+                   _ptr(a["anlga"]), _ptr(a["xnlkb"]), i(a["nanl"]), _ptr(a["hamsm"]), _ptr(a["hdiag"]),
+                   i(a["mxddim"]), i(a["mxdsml"]), i(a["mxdbnd"]), i(a["mxdgve"]), i(a["mxdscr"]), i(a["mxdanl"]))
+    return ifail.value

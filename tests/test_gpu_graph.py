@@ -58,3 +58,52 @@ def test_graph_replays_equal_eager(torch_dev, orc, prec):
     g.close()
     for p in plans:
         p.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_follows_projector_rebuild(torch_dev, prec):
+    """Projectors rebuilt on the device at moved atoms (same count) after the
+    capture: the replay equals an eager apply with the new projectors in both
+    precisions (the FP32 copy is converted again in place).  A rebuild with
+    another projector count makes the launch fail instead of replaying."""
+    torch = torch_dev
+    from cpw2000_amd import hk_psi
+
+    d = synth.make_config(0)
+    pp = synth.kb_pseudo(synth.config_spec(0))
+    plan = hk_psi.Plan.from_inputs(d, projectors=False)
+    rk = np.asarray(d["kpt"], float)
+
+    def build(rat, nkb=pp["nkb"], nanl=pp["nanl"]):
+        plan.build_projectors_kb(rk, nanl, pp["nqnl"], pp["delqnl"], pp["vkb"], nkb, pp["ntype"], pp["natom"],
+                                 rat, pp["adot"])
+
+    build(pp["rat"])
+    x = torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda")
+    out = torch.zeros_like(x)
+    g = hk_psi.Graph([plan], [x], [out], prec)
+    try:
+        g.launch()
+        torch.cuda.synchronize()
+        before = out.clone()
+        moved = pp["rat"].copy(order="F")
+        moved[:, 0, 0] += (0.01, -0.02, 0.015)
+        build(moved)
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        eager = torch.zeros_like(x)
+        plan.apply(x, eager, prec)
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager)
+        assert not torch.equal(before, eager)   # the projectors did change
+        # s projectors only: another count, the launch is refused
+        nkb_s = pp["nkb"].copy(order="F")
+        nkb_s[1] = 0
+        build(moved, nkb=nkb_s, nanl=pp["nanl"] // 4)
+        with pytest.raises(hk_psi.CpwGpuError):
+            g.launch()
+    finally:
+        g.close()
+        plan.close()

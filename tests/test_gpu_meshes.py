@@ -36,7 +36,7 @@ def hk():
 
 def _built(hk, plan, prec):
     out = []
-    for v in range(4):
+    for v in range(hk.NPLANE_VAR):
         try:
             plan.set_variant(prec, hk.TUNE_PLANE, v)
         except hk.CpwGpuError as e:

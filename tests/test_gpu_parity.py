@@ -151,3 +151,26 @@ def test_band_subsets_like_ditsp(hk):
         out = np.zeros_like(sub, order="F")
         hk.hk_psi_c16_dict(d, psi=sub, hpsi=out)
         assert rel(out[: d["mtxd"]], href[:, lo:hi]) < TOL64
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_nan_in_one_band_stays_in_that_band(torch_dev, hk, prec):
+    """A NaN in one coefficient of band 5 (the reference has no check for it,
+    hk_psi_c16.f90:176-227, and returns normally) makes that band's H psi
+    non-finite and leaves every other band exactly as in the clean call: no
+    kernel mixes bands (planes of several bands per block, split-K partials)."""
+    torch = torch_dev
+    d = synth.make_config(0)
+    plan = hk.Plan.from_inputs(d)
+    x = torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda")
+    clean = torch.zeros_like(x)
+    plan.apply(x, clean, prec)
+    bad = x.clone()
+    bad[5, 17] = complex(float("nan"), 0.0)
+    out = torch.zeros_like(x)
+    plan.apply(bad, out, prec)
+    torch.cuda.synchronize()
+    plan.close()
+    assert not torch.isfinite(torch.view_as_real(out[5])).any()
+    others = [b for b in range(x.shape[0]) if b != 5]
+    assert torch.equal(out[others], clean[others])

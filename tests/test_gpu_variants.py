@@ -42,7 +42,7 @@ def hk():
 
 def _variants(hk, plan, prec):
     out = []
-    for v in range(4):
+    for v in range(hk.NPLANE_VAR):
         try:
             plan.set_variant(prec, hk.TUNE_PLANE, v)
         except hk.CpwGpuError as e:
@@ -123,7 +123,7 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
 def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
-    for what, value in ((hk.TUNE_PLANE, 4), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
+    for what, value in ((hk.TUNE_PLANE, hk.NPLANE_VAR), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
                         (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (hk.TUNE_SPIN, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)

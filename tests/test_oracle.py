@@ -151,3 +151,24 @@ def test_error_codes(orc):
     km[3:7] = (10, 10, 10, 10)
     e = dict(d, kmscr=km)
     assert orc.orc_hk_psi(e)[0] == 3
+
+
+def test_nan_in_one_band_stays_in_that_band(orc):
+    """The restatement, like the reference (no check on psi), returns status 0
+    for a NaN in band 5; that band's H psi is non-finite, the others equal
+    the clean call bit for bit."""
+    import numpy as np
+
+    from cpw2000_amd import synth
+
+    d = synth.make_config(0)
+    st, clean = orc.orc_hk_psi(d)
+    assert st == 0
+    psi = d["psi"].copy(order="F")
+    psi[17, 5] = np.nan
+    st, out = orc.orc_hk_psi(d, psi=psi)
+    assert st == 0
+    m = d["mtxd"]
+    assert not np.isfinite(out[:m, 5]).any()
+    others = [b for b in range(psi.shape[1]) if b != 5]
+    assert np.array_equal(out[:m, others], clean[:m, others])
