@@ -2350,6 +2350,85 @@ int tune_spin(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
 
 }  // namespace
 
+// ---- allocation-free apply ----------------------------------------------------
+// Workspace of cpwgpu_plan_apply_noalloc: a fixed device array, one copy per
+// device, held until the process ends.  Host code never passes the array's
+// name to a kernel: it is a host-side stand-in.  The device address of the
+// current device's copy comes from hipGetSymbolAddress (scratch_addr).
+constexpr size_t NOALLOC_SCRATCH = (size_t)CPWGPU_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH];
+
+namespace {
+constexpr int MAXDEV = 64;
+void *g_scratch_ptr[MAXDEV] = {};
+
+// device address of the current device's scratch array; resolved once per device
+int scratch_addr(void **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAXDEV) return fail(CPWGPU_ERR_ARG, "device %d out of range", dev);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(CPWGPU_ERR_HIP, "no device address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CPWGPU_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// bytes of scratch an apply of nb bands takes: z-column buffer, split-K
+// partials and D<beta|psi>, each 256-byte aligned
+template <typename R> size_t noalloc_need(const cpwgpu_plan_s *p, int nb) {
+    const size_t cs = sizeof(cplx<R>);
+    size_t n = align256(cs * (size_t)p->g.n[2] * p->g.ncolp * nb);
+    if (p->nanl > 0 && p->g.mtxd > 0) {
+        int nsl, kslice;
+        splitk_shape<R>(p, p->g.mtxd, p->nanl, nb, &nsl, &kslice);
+        n += align256(cs * (size_t)nsl * nb * p->nanl) + align256(cs * (size_t)nb * p->nanl);
+    }
+    return n;
+}
+
+// H|psi> in band chunks whose workspace fits in the scratch array at scr
+template <typename R>
+int apply_noalloc_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+                    hipStream_t s, unsigned char *scr) {
+    const bool nl = p->nanl > 0 && p->g.mtxd > 0;
+    if (sizeof(R) == 4 && nl && !p->anl32_valid)
+        return fail(CPWGPU_ERR_ARG, "no FP32 copy of the projectors: it is made by the first FP32 "
+                                    "cpwgpu_plan_apply after the projectors are set");
+    if (noalloc_need<R>(p, 1) > NOALLOC_SCRATCH)
+        return fail(CPWGPU_ERR_SIZE, "one band needs %zu bytes of workspace, more than the %zu of the "
+                                     "allocation-free entry point",
+                    noalloc_need<R>(p, 1), NOALLOC_SCRATCH);
+    int nbc = neig;
+    while (nbc > 1 && noalloc_need<R>(p, nbc) > NOALLOC_SCRATCH) nbc--;
+    for (int b0 = 0; b0 < neig; b0 += nbc) {
+        const int nb = std::min(nbc, neig - b0);
+        const size_t cs = sizeof(cplx<R>);
+        Work w;
+        w.colbuf = scr;
+        w.colbuf_n = cs * (size_t)p->g.n[2] * p->g.ncolp * nb;
+        size_t off = align256(w.colbuf_n);
+        if (nl) {
+            int nsl, kslice;
+            splitk_shape<R>(p, p->g.mtxd, p->nanl, nb, &nsl, &kslice);
+            w.part = scr + off;
+            w.part_n = cs * (size_t)nsl * nb * p->nanl;
+            off += align256(w.part_n);
+            w.dhd = scr + off;
+            w.dhd_n = cs * (size_t)nb * p->nanl;
+        }
+        const int st = apply_t<R>(p, nb, psi + 2 * (long)b0 * ldp, ldp, hpsi + 2 * (long)b0 * ldh, ldh, s, &w);
+        if (st) return st;
+    }
+    return CPWGPU_OK;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -2366,6 +2445,9 @@ int cpwgpu_init(int device) {
     }
     g_device = device;
     HIPCHK(hipSetDevice(device));
+    void *scr = nullptr;
+    int st = scratch_addr(&scr);  // the allocation-free entry point's workspace on this device
+    if (st) return st;
     return ensure_stream();
 }
 
@@ -2534,6 +2616,23 @@ int cpwgpu_proj_nl_kb_c16(const double *rkpt, const int *mtxd, const int *isort,
                             hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return CPWGPU_OK;
+}
+
+int cpwgpu_plan_apply_noalloc(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                              double *hpsi_dev, long ldh, int prec, void *stream) {
+    if (!plan) return fail(CPWGPU_ERR_ARG, "null plan");
+    if (neig < 0) return fail(CPWGPU_ERR_ARG, "neig = %d", neig);
+    if (neig == 0) return CPWGPU_OK;
+    if (!psi_dev || !hpsi_dev) return fail(CPWGPU_ERR_ARG, "null psi/hpsi");
+    if (ldp < plan->mtxd || ldh < plan->mtxd) return fail(CPWGPU_ERR_ARG, "leading dimension < mtxd");
+    if (prec != CPWGPU_PREC_FP64 && prec != CPWGPU_PREC_FP32) return fail(CPWGPU_ERR_MODE, "precision %d", prec);
+    void *scr = nullptr;
+    int st = scratch_addr(&scr);
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == CPWGPU_PREC_FP64)
+        return apply_noalloc_t<double>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s, (unsigned char *)scr);
+    return apply_noalloc_t<float>(plan, neig, psi_dev, ldp, hpsi_dev, ldh, s, (unsigned char *)scr);
 }
 
 int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,

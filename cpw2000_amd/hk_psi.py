@@ -51,7 +51,7 @@ EXPORTS = (
     "cpwgpu_plan_ditsp", "cpwgpu_ditsp_c16", "cpwgpu_plan_density",
     "cpwgpu_hk_psi_spin_c16", "cpwgpu_plan_set_potential_spin", "cpwgpu_plan_set_projectors_spin",
     "cpwgpu_plan_apply_spin", "cpwgpu_hk_psi_c8", "cpwgpu_hk_psi_allk_c16",
-    "cpwgpu_proj_nl_kb_c16", "cpwgpu_plan_build_projectors_kb",
+    "cpwgpu_proj_nl_kb_c16", "cpwgpu_plan_build_projectors_kb", "cpwgpu_plan_apply_noalloc",
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
@@ -101,6 +101,7 @@ def lib():
         L.cpwgpu_plan_build_projectors_kb.argtypes = [vp, dp, C.c_int, vp, dp, dp, vp, C.c_int, vp, dp, dp,
                                                       C.c_int, C.c_int, C.c_int]
         L.cpwgpu_plan_apply.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
+        L.cpwgpu_plan_apply_noalloc.argtypes = [vp, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int, vp]
         L.cpwgpu_plan_info.argtypes = [vp, ip]
         L.cpwgpu_profile.argtypes = [C.c_int, C.c_uint]
         L.cpwgpu_rayleigh_ritz.argtypes = [C.c_int, C.c_int, dp, C.c_long, dp, C.c_long, C.c_int,
@@ -483,6 +484,19 @@ class Plan:
         _check(lib().cpwgpu_plan_apply(self._h, int(neig), C.c_void_p(psi_t.data_ptr()),
                                        _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
                                        int(prec), C.c_void_p(stream)))
+        return hpsi_t
+
+    def apply_noalloc(self, psi_t, hpsi_t, prec=PREC_FP64, stream=None):
+        """apply() with the library's fixed per-device workspace instead of the
+        plan's buffers: no device allocation (cpwgpu_plan_apply_noalloc)."""
+        import torch
+
+        neig = psi_t.shape[0]
+        if stream is None:
+            stream = torch.cuda.current_stream(psi_t.device).cuda_stream
+        _check(lib().cpwgpu_plan_apply_noalloc(self._h, int(neig), C.c_void_p(psi_t.data_ptr()),
+                                               _ld(psi_t), C.c_void_p(hpsi_t.data_ptr()), _ld(hpsi_t),
+                                               int(prec), C.c_void_p(stream)))
         return hpsi_t
 
     def set_potential_spin(self, vscr_sp, nsp):

@@ -164,6 +164,20 @@ int cpwgpu_plan_build_projectors_kb(cpwgpu_plan_t plan, const double *rkpt, int 
  * stream).  Returns when the work is enqueued. */
 int cpwgpu_plan_apply(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
                       double *hpsi_dev, long ldh, int prec, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same H|psi> as cpwgpu_plan_apply, with the z-column
+ * buffer, split-K partials and D<beta|psi> in a fixed device array of
+ * CPWGPU_NOALLOC_SCRATCH_MIB MiB per device (held until the process ends)
+ * instead of the plan's buffers.  Bands go through in chunks that fit; a
+ * plan whose single band needs more returns CPWGPU_ERR_SIZE.  No hipMalloc,
+ * hipFree or allocating library call.  FP32 needs the plan's FP32 projector
+ * copy, made by one FP32 cpwgpu_plan_apply after the projectors are set
+ * (else CPWGPU_ERR_ARG).  Every call on a device shares that device's
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+#define CPWGPU_NOALLOC_SCRATCH_MIB 64
+int cpwgpu_plan_apply_noalloc(cpwgpu_plan_t plan, int neig, const double *psi_dev, long ldp,
+                              double *hpsi_dev, long ldh, int prec, void *stream);
 /* Two-component bands on the same plan (the device side of
  * cpwgpu_hk_psi_spin_c16): the potential components vscr_sp(mxdscr, nsp) and
  * the projectors anlsp(2*mxddim, nanlsp) from host memory, then H psi_sp on
