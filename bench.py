@@ -282,7 +282,8 @@ def bench_kpoints(torch, hk, synth, np, steps):
         c = plans[0].tune(psis[0], outs[0], pid)
         for p in plans[1:]:
             for what, key in ((hk.TUNE_PLANE, "plane"), (hk.TUNE_PROJ, "proj"),
-                              (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk"), (hk.TUNE_KIN, "kin")):
+                              (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk"), (hk.TUNE_KIN, "kin"),
+                              (hk.TUNE_COL, "col")):
                 p.set_variant(pid, what, c[key])
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -636,7 +637,7 @@ def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
     out = {}
     for prec, pid in (("fp32", hk.PREC_FP32), ("fp64", hk.PREC_FP64)):
         work = kernel_work(info, nanl, prec)
-        for what, fam, nv in ((hk.TUNE_PLANE, "plane", 4), (hk.TUNE_PROJ, "proj", 6),
+        for what, fam, nv in ((hk.TUNE_PLANE, "plane", hk.NPLANE_VAR), (hk.TUNE_PROJ, "proj", 6),
                               (hk.TUNE_BACK, "back", 5)):
             for v in range(nv):
                 try:
@@ -658,6 +659,28 @@ def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
                                            "unit": e["unit"], "frac": e["frac"]}
                 log(f"bench: {prec} {fam} variant {v}: {out[f'{prec}/{fam}{v}']}")
             plan.set_variant(pid, what, -1)
+        for v in (0, 1):  # column kernels: LDS lines, register lines
+            try:
+                plan.set_variant(pid, hk.TUNE_COL, v)
+            except hk.CpwGpuError:
+                continue
+            for _ in range(2):
+                plan.apply(psi, h, pid)
+            torch.cuda.synchronize()
+            hk.profile(True, ["k_col_bwd", "k_col_fwd"])
+            hk.profile_read()
+            for _ in range(reps):
+                plan.apply(psi, h, pid)
+            torch.cuda.synchronize()
+            r = hk.profile_read()
+            hk.profile(False)
+            for k in ("k_col_bwd", "k_col_fwd"):
+                ms, cnt = r[k]
+                e = roofline_entry(k, work, ms / max(cnt, 1), nb, prec)
+                out[f"{prec}/{k}{v}"] = {"us": round(1e3 * ms / max(cnt, 1), 1), "achieved": e["achieved"],
+                                         "unit": e["unit"], "frac": e["frac"]}
+                log(f"bench: {prec} {k} variant {v}: {out[f'{prec}/{k}{v}']}")
+        plan.set_variant(pid, hk.TUNE_COL, -1)
     hk.set_overlap(True)
     x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
     x.fill_(1.0)

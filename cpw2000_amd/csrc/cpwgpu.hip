@@ -24,6 +24,7 @@
 #include "fft_local.hpp"
 #include "fft_reg.hpp"
 #include "fft_wave.hpp"
+#include "fft_col_reg.hpp"
 #include "nl_gemm.hpp"
 #include "subspace.hpp"
 #include "forces.hpp"
@@ -178,6 +179,7 @@ struct cpwgpu_plan_s {
     std::vector<double> h_ekpg;
     // device arrays
     DevBuf col_ptr, col_uy, ent_i, ent_cz, ekpg, tw[3];
+    DevBuf zmap;              // [col][z] -> psi row or -1 (register column kernels)
     DevBuf v64, v32;          // V(r) in FP64 and FP32
     double v0 = 0.0;          // mean of V(r) over the mesh = V(G=0), the local part of diag(H)
     DevBuf anl64, anl32, xnl; // projectors
@@ -187,6 +189,8 @@ struct cpwgpu_plan_s {
     // plane-stage kernel per precision (index 0 FP64, 1 FP32): -1 = default for
     // the mesh, else a variant number (set by the tuner or by the caller)
     int plane_var[2] = {-1, -1};
+    // column kernels: 0 LDS lines, 1 register lines; -1 default
+    int col_var[2] = {-1, -1};
     // split-K target blocks of the projection GEMM per precision (-1 = 256)
     int splitk[2] = {-1, -1};
     // projection GEMM kernel per precision: 0 LDS-staged k_proj, 1 k_proj_direct
@@ -327,6 +331,11 @@ int build_plan(cpwgpu_plan_s *p, int mtxd, int ng, const int *kgv, const double 
     if ((st = upload(p->col_uy, uy.data(), uy.size()))) return st;
     if ((st = upload(p->ent_i, ei.data(), ei.size()))) return st;
     if ((st = upload(p->ent_cz, ecz.data(), ecz.size()))) return st;
+    {
+        std::vector<int> zm((size_t)std::max(ncol, 1) * ns[2], -1);
+        for (int i = 0; i < mtxd; i++) zm[(size_t)colid[(long)ky[i] * ns[0] + kx[i]] * ns[2] + kz[i]] = i;
+        if ((st = upload(p->zmap, zm.data(), zm.size()))) return st;
+    }
     p->h_ekpg.assign(ekpg, ekpg + mtxd);
     if ((st = upload(p->ekpg, ekpg, mtxd))) return st;
     for (int j = 0; j < 3; j++) {
@@ -409,23 +418,42 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
 }
 
 // ---- plane stage variants ---------------------------------------------------
-// 0: k_plane_fast (LDS two-pass lines); 1..3: k_plane_reg shapes
-// (x lanes GX, y lanes GY, planes per block P, threads T) per precision.
-template <typename R> struct RegShape;
-template <> struct RegShape<float> {
-    // spill-free at 72: 67, 74 and 127 VGPRs
-    static constexpr int GX[3] = {8, 8, 4}, GY[3] = {8, 8, 4}, P[3] = {1, 2, 1};
-    static constexpr int T[3] = {256, 256, 256}, W[3] = {5, 4, 3};
+// 0: k_plane_fast (LDS two-pass lines); 4..6: k_plane_wave (LDS two-pass
+// lines, wave-local exchanges) with 256, 192 and 128 threads per block; the
+// others: k_plane_reg shapes (x lanes GX, y lanes GY, planes per block P,
+// threads T, waves per SIMD W, y lines through the LDS wave transform YW,
+// box pruning PR) per precision.
+struct RegShapeT {
+    int GX, GY, P, T, W;
+    bool YW, PR;
 };
-template <> struct RegShape<double> {
-    static constexpr int GX[3] = {8, 8, 4}, GY[3] = {8, 8, 8}, P[3] = {1, 1, 1};
-    static constexpr int T[3] = {256, 320, 256}, W[3] = {3, 4, 3};
-};
-// 4..6: k_plane_wave (LDS two-pass lines, wave-local exchanges) with
-// 256, 192 and 128 threads per block; 7: register shape 1 for the x rows with
-// the y lines through LDS (wave-local exchanges)
-constexpr int NPLANE_VAR = 8;
+template <typename R> constexpr RegShapeT reg_shape(int v);
+template <> constexpr RegShapeT reg_shape<float>(int v) {
+    // 1-3 spill-free at 72: 67, 74 and 127 VGPRs
+    return v == 1 ? RegShapeT{8, 8, 1, 256, 5, false, false}
+         : v == 2 ? RegShapeT{8, 8, 2, 256, 4, false, false}
+         : v == 3 ? RegShapeT{4, 4, 1, 256, 3, false, false}
+         : v == 7 ? RegShapeT{8, 8, 1, 256, 5, true, false}
+         : v == 8 ? RegShapeT{2, 2, 1, 256, 3, true, true}
+         : v == 9 ? RegShapeT{4, 4, 1, 256, 4, true, true}
+         : v == 10 ? RegShapeT{8, 8, 1, 256, 5, true, true}
+         : v == 11 ? RegShapeT{2, 4, 1, 256, 3, false, true}
+                   : RegShapeT{0, 0, 0, 0, 0, false, false};
+}
+template <> constexpr RegShapeT reg_shape<double>(int v) {
+    return v == 1 ? RegShapeT{8, 8, 1, 256, 3, false, false}
+         : v == 2 ? RegShapeT{8, 8, 1, 320, 4, false, false}
+         : v == 3 ? RegShapeT{4, 8, 1, 256, 3, false, false}
+         : v == 7 ? RegShapeT{8, 8, 1, 256, 3, true, false}
+         : v == 8 ? RegShapeT{4, 4, 1, 256, 3, true, true}
+         : v == 9 ? RegShapeT{8, 8, 1, 256, 3, true, true}
+         : v == 10 ? RegShapeT{4, 4, 1, 256, 3, false, true}
+         : v == 11 ? RegShapeT{8, 8, 1, 256, 3, false, true}
+                   : RegShapeT{0, 0, 0, 0, 0, false, false};
+}
+constexpr int NPLANE_VAR = 12;
 constexpr int WAVE_T[3] = {256, 192, 128};
+constexpr bool is_reg_variant(int v) { return (v >= 1 && v <= 3) || (v >= 7 && v < NPLANE_VAR); }
 // back-projection kernels: 0 LDS-staged, 1..NBACK_DIRECT direct, NBACK_DIRECT+1 LDS-staged with half chunks
 constexpr int NBACK_VAR = NBACK_DIRECT + 2;
 // projection kernels: 0 LDS-staged, 1..NPROJ_DIRECT direct, NPROJ_DIRECT+1 LDS-staged with half wave tiles
@@ -436,34 +464,47 @@ template <typename R> constexpr int proj_half_wn() { return GemmCfg<R>::WN > 1 ?
 // a register shape is built for a mesh length when both splits divide N and a
 // lane holds at most 288 bytes of one line
 template <typename R, int N, int GX, int GY> constexpr bool reg_ok() {
-    return N % GX == 0 && N % GY == 0 && first_factor(N / GX) <= 5 && first_factor(N / GY) <= 5 &&
-           2 * sizeof(R) * (N / GX) <= 288 && 2 * sizeof(R) * (N / GY) <= 288;
+    return GX > 0 && GY > 0 && N % GX == 0 && N % GY == 0 && first_factor(N / GX) <= 5 &&
+           first_factor(N / GY) <= 5 && 2 * sizeof(R) * (N / GX) <= 288 && 2 * sizeof(R) * (N / GY) <= 288 &&
+           N / GX <= 64 && N / GY <= 64;
 }
+template <typename R, int N> constexpr bool reg_var_ok(int v) {
+#define CPW_RV(V) (v == V && reg_ok<R, N, reg_shape<R>(V).GX, reg_shape<R>(V).GY>())
+    return CPW_RV(1) || CPW_RV(2) || CPW_RV(3) || CPW_RV(7) || CPW_RV(8) || CPW_RV(9) || CPW_RV(10) ||
+           CPW_RV(11);
+#undef CPW_RV
+}
+// a pruned shape needs the sphere box inside |x|, |y| <= N/4 (true on the
+// meshes size_fft chooses for the density, n >= 4 kd + 1)
+inline bool prune_fits(const FftGeom &g) { return 4 * g.kd[0] <= g.n[0] && 4 * g.kd[1] <= g.n[1]; }
 
 // LDS row stride of the register variant's sub-plane for nx box columns
 inline int reg_stride(int nx) { return plane_sub_stride(nx); }
 
-template <typename R, int N, int V, bool YW = false>
+template <typename R, int N, int V>
 int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr,
                      hipStream_t s) {
-    constexpr int GX = RegShape<R>::GX[V - 1], GY = RegShape<R>::GY[V - 1];
-    constexpr int P = RegShape<R>::P[V - 1], T = RegShape<R>::T[V - 1], W = RegShape<R>::W[V - 1];
+    constexpr RegShapeT SH = reg_shape<R>(V);
+    constexpr int GX = SH.GX, GY = SH.GY, P = SH.P, T = SH.T, W = SH.W;
+    constexpr bool YW = SH.YW, PR = SH.PR;
     if constexpr (!reg_ok<R, N, GX, GY>()) {
         return fail(CPWGPU_ERR_MODE, "plane variant %d is not built for N = %d", V, N);
     } else {
         const FftGeom &g = p->g;
+        if (PR && !prune_fits(g))
+            return fail(CPWGPU_ERR_MODE, "plane variant %d needs the sphere box inside N/4", V);
         const int S = reg_stride(g.nx);
         const size_t dyn = plane_reg_lds<R, N, P>(S);
         if (dyn > 160 * 1024)
             return fail(CPWGPU_ERR_SIZE, "plane variant %d: %zu bytes of LDS", V, dyn);
         static bool attr = false;
         if (!attr) {
-            HIPCHK(hipFuncSetAttribute((const void *)k_plane_reg<R, N, GX, GY, P, T, W, YW>,
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_reg<R, N, GX, GY, P, T, W, YW, PR>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
         ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W, YW>), dim3(((a.nb + P - 1) / P) * g.n[2]),
+        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W, YW, PR>), dim3(((a.nb + P - 1) / P) * g.n[2]),
                            dim3(T), dyn, s, colbuf, (const int *)p->col_uy.p, vr,
                            (const double2 *)p->tw[0].p, g, a.nb, S);
         HIPCHK(hipGetLastError());
@@ -491,6 +532,52 @@ int launch_plane_wave(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, con
     return CPWGPU_OK;
 }
 
+// ---- register column kernels -------------------------------------------------
+// lanes per z column: 8 where the column fits in registers, else 4
+template <typename R, int N> constexpr int col_reg_g() {
+    return reg_ok<R, N, 8, 8>() ? 8 : reg_ok<R, N, 4, 4>() ? 4 : 0;
+}
+template <typename R, int N> constexpr bool col_reg_built() { return col_reg_g<R, N>() > 0; }
+// the sphere box inside |z| <= N3/4: the pruned register column kernels
+inline bool col_prune_fits(const FftGeom &g) { return 4 * g.kd[2] <= g.n[2]; }
+
+template <typename R, int N>
+int launch_col_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, int stage, hipStream_t s) {
+    constexpr int G = col_reg_g<R, N>();
+    if constexpr (G == 0) {
+        return fail(CPWGPU_ERR_MODE, "register column kernels are not built for N = %d", N);
+    } else {
+        const FftGeom &g = p->g;
+        constexpr int CPB = COLR_T / G;
+        const int ntiles = (g.ncol + CPB - 1) / CPB;
+        const dim3 grid(8 * ((a.nb + 7) / 8) * ntiles);
+        const bool pr = col_prune_fits(g);
+        const double2 *tw3 = (const double2 *)p->tw[2].p;
+        const int *zm = (const int *)p->zmap.p;
+        if (stage == ST_COL_BWD) {
+            ProfScope ps_(CPWGPU_K_COL_BWD, s);
+            if (pr)
+                hipLaunchKernelGGL((k_col_bwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
+                                   zm, tw3, g, ntiles, a.nb);
+            else
+                hipLaunchKernelGGL((k_col_bwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
+                                   zm, tw3, g, ntiles, a.nb);
+        } else {
+            ProfScope ps_(CPWGPU_K_COL_FWD, s);
+            if (pr)
+                hipLaunchKernelGGL((k_col_fwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
+                                   a.ldh, (const cplx<R> *)colbuf, zm, (const double *)p->ekpg.p, tw3, g, ntiles,
+                                   a.nb, p->inv_n, a.mode);
+            else
+                hipLaunchKernelGGL((k_col_fwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
+                                   a.ldh, (const cplx<R> *)colbuf, zm, (const double *)p->ekpg.p, tw3, g, ntiles,
+                                   a.nb, p->inv_n, a.mode);
+        }
+        HIPCHK(hipGetLastError());
+        return CPWGPU_OK;
+    }
+}
+
 // whether the wave-exchange variant v (4..6) fits in LDS for nx box columns
 template <typename R, int N> bool plane_wave_fits(int v, int nx) {
     const size_t stat = sizeof(cplx<R>) * 2 * N;
@@ -500,10 +587,7 @@ template <typename R, int N> bool plane_wave_fits(int v, int nx) {
 }
 
 template <typename R, int N> constexpr bool plane_var_ok(int v) {
-    return v == 0 || (v == 1 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>()) ||
-           (v == 2 && reg_ok<R, N, RegShape<R>::GX[1], RegShape<R>::GY[1]>()) ||
-           (v == 3 && reg_ok<R, N, RegShape<R>::GX[2], RegShape<R>::GY[2]>()) || (v >= 4 && v <= 6) ||
-           (v == 7 && reg_ok<R, N, RegShape<R>::GX[0], RegShape<R>::GY[0]>());
+    return v == 0 || (v >= 4 && v <= 6) || reg_var_ok<R, N>(v);
 }
 
 // default plane variant of a mesh length
@@ -513,13 +597,12 @@ template <typename R, int N>
 int launch_plane(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *vr, hipStream_t s) {
     int v = p->plane_var[sizeof(R) == 4 ? 1 : 0];
     if (v < 0 || !plane_var_ok<R, N>(v)) v = plane_var_default<R, N>();
-    if (v == 1) return launch_plane_reg<R, N, 1>(p, a, colbuf, vr, s);
-    if (v == 2) return launch_plane_reg<R, N, 2>(p, a, colbuf, vr, s);
-    if (v == 3) return launch_plane_reg<R, N, 3>(p, a, colbuf, vr, s);
+#define CPW_LR(V) if (v == V) return launch_plane_reg<R, N, V>(p, a, colbuf, vr, s);
+    CPW_LR(1) CPW_LR(2) CPW_LR(3) CPW_LR(7) CPW_LR(8) CPW_LR(9) CPW_LR(10) CPW_LR(11)
+#undef CPW_LR
     if (v == 4) return launch_plane_wave<R, N, WAVE_T[0]>(p, a, colbuf, vr, s);
     if (v == 5) return launch_plane_wave<R, N, WAVE_T[1]>(p, a, colbuf, vr, s);
     if (v == 6) return launch_plane_wave<R, N, WAVE_T[2]>(p, a, colbuf, vr, s);
-    if (v == 7) return launch_plane_reg<R, N, 1, true>(p, a, colbuf, vr, s);
     return launch_fast<R, N>(p, a, colbuf, vr, ST_PLANE, s);
 }
 
@@ -541,11 +624,28 @@ template <typename R> int plane_var_resolve(const cpwgpu_plan_s *p, int v, bool 
 #define CPW_PV(N)                                      \
     if (g.n[0] == N) {                                 \
         def = plane_var_default<R, N>();               \
-        if (v > 0) *ok = v < NPLANE_VAR && plane_var_ok<R, N>(v) && (v < 4 || v > 6 || plane_wave_fits<R, N>(v, g.nx)); \
+        if (v > 0)                                     \
+            *ok = v < NPLANE_VAR && plane_var_ok<R, N>(v) && (v < 4 || v > 6 || plane_wave_fits<R, N>(v, g.nx)) && \
+                  (!is_reg_variant(v) || !reg_shape<R>(v).PR || prune_fits(g)); \
     }
     CPW_FAST_SIZES(CPW_PV)
 #undef CPW_PV
     return v < 0 || !*ok ? def : v;
+}
+
+// column kernel the next apply uses (default resolved: 0, the LDS lines,
+// which measured as fast as the register lines in FP64 and faster in FP32 at
+// 72^3); *ok: whether v is usable (1 needs the register kernels built for n3)
+template <typename R> int col_var_resolve(const cpwgpu_plan_s *p, int v, bool *ok) {
+    const FftGeom &g = p->g;
+    bool built = false;
+    if (fast_sizes(g)) {
+#define CPW_CB(N) if (g.n[2] == N) built = col_reg_built<R, N>();
+        CPW_FAST_SIZES(CPW_CB)
+#undef CPW_CB
+    }
+    if (ok) *ok = v <= 0 || (v == 1 && built);
+    return v == 1 && built ? 1 : 0;
 }
 
 // One stage of the local term: specialised kernels when the mesh allows,
@@ -562,7 +662,10 @@ int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
             CPW_FAST_SIZES(CPW_PL)
 #undef CPW_PL
         } else {
-#define CPW_CO(N) if (g.n[2] == N) st = launch_fast<R, N>(p, a, colbuf, vr, stage, s);
+            const bool reg = col_var_resolve<R>(p, p->col_var[sizeof(R) == 4 ? 1 : 0], nullptr) == 1;
+#define CPW_CO(N)                                                                \
+    if (g.n[2] == N)                                                             \
+        st = reg ? launch_col_reg<R, N>(p, a, colbuf, stage, s) : launch_fast<R, N>(p, a, colbuf, vr, stage, s);
             CPW_FAST_SIZES(CPW_CO)
 #undef CPW_CO
         }
@@ -2165,9 +2268,9 @@ struct TuneKey {
     }
 };
 
-// one line per key: prec n1 n2 n3 nx ncol nanl neig | plane proj back splitk kin
-// (lines without the last field, kin, read it as -1)
-bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk, int *kin) {
+// one line per key: prec n1 n2 n3 nx ncol nanl neig | plane proj back splitk kin col
+// (lines without the last fields, kin and col, read them as -1)
+bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk, int *kin, int *col) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return false;
     FILE *f = std::fopen(path, "r");
@@ -2176,16 +2279,17 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
     bool found = false;
     while (std::fgets(line, sizeof line, f)) {
         TuneKey r;
-        int pv, jv, bv, sk, kv = -1;
-        const int n = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2, &r.n3,
-                                  &r.nx, &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk, &kv);
+        int pv, jv, bv, sk, kv = -1, cv = -1;
+        const int n = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2,
+                                  &r.n3, &r.nx, &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk, &kv, &cv);
         if (n < 12) continue;
         if (r == k) {  // the last entry for a key wins
             *plane = pv;
             *proj = jv;
             *back = bv;
             *splitk = sk;
-            *kin = n == 13 ? kv : -1;
+            *kin = n >= 13 ? kv : -1;
+            *col = n >= 14 ? cv : -1;
             found = true;
         }
     }
@@ -2193,13 +2297,13 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
     return found;
 }
 
-void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk, int kin) {
+void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk, int kin, int col) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
-                 k.nanl, k.neig, plane, proj, back, splitk, kin);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
+                 k.nanl, k.neig, plane, proj, back, splitk, kin, col);
     std::fclose(f);
 }
 
@@ -2235,8 +2339,8 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
     const FftGeom &g = p->g;
     const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
-    int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1;
-    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv)) {
+    int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1, cv = -1;
+    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv, &cv)) {
         bool ok;
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
@@ -2245,10 +2349,27 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         p->back_var[ix] = bv >= 0 && bv < NBACK_VAR ? bv : -1;
         p->splitk[ix] = sk >= 64 && sk <= 4096 ? sk : -1;
         p->kin_var[ix] = kv == 0 || kv == 1 ? kv : -1;
+        bool cok;
+        if (prec == CPWGPU_PREC_FP32) col_var_resolve<float>(p, cv, &cok);
+        else col_var_resolve<double>(p, cv, &cok);
+        p->col_var[ix] = cok && cv >= 0 ? cv : -1;
         return cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
     }
     int st;
     float best = 3.4e38f, t;
+    // column kernels (with the default plane kernel), then the plane kernels
+    int best_cv = 0;
+    for (int cv2 = 0; cv2 <= 1; cv2++) {
+        bool ok;
+        if (prec == CPWGPU_PREC_FP32) col_var_resolve<float>(p, cv2, &ok);
+        else col_var_resolve<double>(p, cv2, &ok);
+        if (!ok) continue;
+        p->col_var[ix] = cv2;
+        if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+        if (t < best) { best = t; best_cv = cv2; }
+    }
+    p->col_var[ix] = best_cv;
+    best = 3.4e38f;
     int best_pv = 0;
     for (int v = 0; v < NPLANE_VAR; v++) {
         bool ok;
@@ -2298,7 +2419,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     p->kin_var[ix] = best_kv;
     // leave hpsi holding H psi of the chosen configuration
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
-    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv);
+    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv, best_cv);
     return CPWGPU_OK;
 }
 
@@ -2876,6 +2997,14 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         plan->spin_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
+    if (what == CPWGPU_TUNE_COL) {
+        bool ok = true;
+        if (prec == CPWGPU_PREC_FP32) col_var_resolve<float>(plan, value, &ok);
+        else col_var_resolve<double>(plan, value, &ok);
+        if (value > 1 || !ok) return fail(CPWGPU_ERR_MODE, "column kernel %d is not built for this mesh", value);
+        plan->col_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
     if (what == CPWGPU_TUNE_SPLITK) {
         if (value >= 0 && (value < 64 || value > 4096))
             return fail(CPWGPU_ERR_ARG, "split-K target %d outside 64..4096", value);
@@ -2913,6 +3042,11 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
     }
     if (what == CPWGPU_TUNE_SPIN) {
         *value = prec == CPWGPU_PREC_FP32 ? spin_var_resolve<float>(plan) : spin_var_resolve<double>(plan);
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_COL) {
+        *value = prec == CPWGPU_PREC_FP32 ? col_var_resolve<float>(plan, plan->col_var[ix], nullptr)
+                                          : col_var_resolve<double>(plan, plan->col_var[ix], nullptr);
         return CPWGPU_OK;
     }
     return fail(CPWGPU_ERR_ARG, "unknown tunable %d", what);

@@ -16,6 +16,8 @@
 // two passes from the distribution pass 2 leaves behind.
 #pragma once
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define CPW_HD __host__ __device__ __forceinline__
@@ -186,6 +188,158 @@ template <int M, int DIR, typename R> struct Dft {
 };
 
 template <int M, int DIR, typename R> CPW_HD void dft(cplx<R> *v) { Dft<M, DIR, R>::run(v); }
+
+// ---------------------------------------------------------------------------
+// pruned DFTs
+//
+//   DftP<M, DIR, R, IN, OUT>::run(v): the transform of dft<M, DIR> when the
+//   inputs outside the bit mask IN are zero (they are never read) and only the
+//   outputs in the bit mask OUT are used (the others are left undefined).
+//   M <= 64.  Zero sub-transforms are skipped, zero terms are left out of the
+//   sums, and the work behind unused outputs is never written, so the compiler
+//   drops it.  Quarter-turn twiddles are exact rotations, not products.
+// ---------------------------------------------------------------------------
+typedef unsigned long long u64mask;
+
+constexpr u64mask full_mask(int m) { return m >= 64 ? ~0ull : (1ull << m) - 1; }
+constexpr bool has_bit(u64mask m, int b) { return (m >> b) & 1ull; }
+// bits t < Q with s + P t in mask
+constexpr u64mask stride_mask(u64mask m, int s, int P, int Q) {
+    u64mask r = 0;
+    for (int t = 0; t < Q; t++)
+        if (has_bit(m, s + P * t)) r |= 1ull << t;
+    return r;
+}
+// bits s < P whose stride-P sub-sequence has a bit in mask
+constexpr u64mask live_subs(u64mask m, int P, int Q) {
+    u64mask r = 0;
+    for (int s = 0; s < P; s++)
+        if (stride_mask(m, s, P, Q)) r |= 1ull << s;
+    return r;
+}
+// bits k < Q with some k + Q q (q < P) in mask
+constexpr u64mask needed_cols(u64mask m, int P, int Q) {
+    u64mask r = 0;
+    for (int k = 0; k < Q; k++)
+        for (int q = 0; q < P; q++)
+            if (has_bit(m, k + Q * q)) r |= 1ull << k;
+    return r;
+}
+
+constexpr int lowest_bit(u64mask m) {
+    int b = 0;
+    while (b < 64 && !has_bit(m, b)) b++;
+    return b;
+}
+
+template <int I, int N, typename F> CPW_HD void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <int M> struct RootTable {
+    static constexpr Roots<M> W{};
+};
+
+// a * exp(DIR 2 pi i E / M), E a compile-time exponent
+template <int M, int E, int DIR, typename R> CPW_HD cplx<R> twiddle(cplx<R> a) {
+    constexpr int e = ((E % M) + M) % M;
+    if constexpr (e == 0) {
+        return a;
+    } else if constexpr (2 * e == M) {
+        return mk<R>(-a.x, -a.y);
+    } else if constexpr (4 * e == M) {
+        return rot(a, DIR);
+    } else if constexpr (4 * e == 3 * M) {
+        return rot(a, -DIR);
+    } else {
+        return cmul(a, mk<R>((R)RootTable<M>::W.c[e], (R)(DIR * RootTable<M>::W.s[e])));
+    }
+}
+
+// leaf: direct sums over the nonzero inputs (the optimised leaf when all are)
+template <int P, int DIR, typename R, u64mask IN> CPW_HD void dft_leaf_masked(cplx<R> *u) {
+    if constexpr (IN == full_mask(P)) {
+        Dft<P, DIR, R>::run(u);
+    } else {
+        cplx<R> out[P];
+        static_for<0, P>([&](auto q_) {
+            constexpr int q = decltype(q_)::value;
+            cplx<R> acc = mk<R>(0, 0);
+            static_for<0, P>([&](auto s_) {
+                constexpr int s = decltype(s_)::value;
+                if constexpr (has_bit(IN, s)) {
+                    const cplx<R> t = twiddle<P, s * q, DIR, R>(u[s]);
+                    if constexpr (s == lowest_bit(IN)) acc = t;
+                    else acc = acc + t;
+                }
+            });
+            out[q] = acc;
+        });
+#pragma unroll
+        for (int q = 0; q < P; q++) u[q] = out[q];
+    }
+}
+
+template <int M, int DIR, typename R, u64mask IN, u64mask OUT> struct DftP {
+    static CPW_HD void run(cplx<R> *v) {
+        static_assert(M <= 64, "pruned DFT longer than 64");
+        if constexpr (IN == full_mask(M) && OUT == full_mask(M)) {
+            Dft<M, DIR, R>::run(v);
+        } else if constexpr (M <= 5) {
+            dft_leaf_masked<M, DIR, R, IN>(v);
+        } else {
+            constexpr int P = first_factor(M);
+            constexpr int Q = M / P;
+            constexpr u64mask KNEED = needed_cols(OUT, P, Q);  // stage-2 transforms that are used
+            constexpr u64mask SLIVE = live_subs(IN, P, Q);     // stage-1 transforms with nonzero input
+            cplx<R> t[M];
+            static_for<0, P>([&](auto s_) {
+                constexpr int s = decltype(s_)::value;
+                constexpr u64mask SIN = stride_mask(IN, s, P, Q);
+                if constexpr (SIN != 0) {
+                    cplx<R> u[Q];
+#pragma unroll
+                    for (int j = 0; j < Q; j++)
+                        if (has_bit(SIN, j)) u[j] = v[s + P * j];
+                    DftP<Q, DIR, R, SIN, KNEED>::run(u);
+                    static_for<0, Q>([&](auto k_) {
+                        constexpr int k = decltype(k_)::value;
+                        if constexpr (has_bit(KNEED, k)) t[s * Q + k] = twiddle<M, s * k, DIR, R>(u[k]);
+                    });
+                }
+            });
+            static_for<0, Q>([&](auto k_) {
+                constexpr int k = decltype(k_)::value;
+                if constexpr (has_bit(KNEED, k)) {
+                    constexpr u64mask QOUT = stride_mask(OUT, k, Q, P);
+                    cplx<R> u[P];
+#pragma unroll
+                    for (int s = 0; s < P; s++)
+                        if (has_bit(SLIVE, s)) u[s] = t[s * Q + k];
+                    DftP<P, DIR, R, SLIVE, QOUT>::run(u);
+#pragma unroll
+                    for (int q = 0; q < P; q++)
+                        if (has_bit(QOUT, q)) v[k + Q * q] = u[q];
+                }
+            });
+        }
+    }
+};
+
+// bit mask of the slots j < M (x = g + G j, every lane position g < G) whose
+// x lies inside the box |x| <= KD of a length-N line
+constexpr u64mask box_slots(int N, int G, int M, int KD) {
+    u64mask r = 0;
+    for (int j = 0; j < M; j++)
+        for (int g = 0; g < G; g++) {
+            const int x = g + G * j;
+            if (x <= KD || x >= N - KD) r |= 1ull << j;
+        }
+    return r;
+}
 
 // Split of a line length N into A*B for the two-pass transform (A <= B, both
 // register-sized); 0 if N is not served by the specialised kernels.

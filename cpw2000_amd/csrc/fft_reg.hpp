@@ -96,10 +96,12 @@ template <int DIR, typename R> __device__ __forceinline__ cplx<R> cmul_dir(cplx<
 }
 
 // DIF transform of a G-lane line (layouts above); twN[e] = exp(+2 pi i e/N).
-template <int DIR, int G, int M, typename R>
+// IN: the register slots j that can hold a nonzero value (the others are zero
+// on every lane and are not read).
+template <int DIR, int G, int M, typename R, u64mask IN = full_mask(M)>
 __device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
                                          const cplx<R> *twN) {
-    dft<M, DIR, R>(v);
+    DftP<M, DIR, R, IN, full_mask(M)>::run(v);
     if constexpr (G > 1) {
         const int gi = opaque(L.g);
 #pragma unroll
@@ -118,7 +120,8 @@ __device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
 }
 
 // DIT transform of a G-lane line, the transpose of line_dif's structure.
-template <int DIR, int G, int M, typename R>
+// OUT: the output slots m the caller uses (the others are left undefined).
+template <int DIR, int G, int M, typename R, u64mask OUT = full_mask(M)>
 __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
                                          const cplx<R> *twN) {
     if constexpr (G > 1) {
@@ -136,7 +139,7 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 #pragma unroll
         for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], twN[gi * k]);
     }
-    dft<M, DIR, R>(v);
+    DftP<M, DIR, R, full_mask(M), OUT>::run(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -148,8 +151,12 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 //   x, V(r), forward x (GX-lane lines, no barrier) -> forward y -> scatter.
 // YW: the y lines instead go through the LDS two-pass transform in place in
 // sub, each line exchanged within one wavefront (line_fft_wave; GY unused).
+// PR: the sphere box is |x|, |y| <= N/4 (checked by the caller), so the
+// register slots that hold only x (y) outside the box on every lane are known
+// zeros of the inverse transforms and unused outputs of the forward ones:
+// they are neither loaded, transformed nor stored.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int GX, int GY, int P, int T, int W, bool YW = false>
+template <typename R, int N, int GX, int GY, int P, int T, int W, bool YW = false, bool PR = false>
 __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf,
                                                  const int *__restrict__ col_uy,
                                                  const R *__restrict__ vr,
@@ -157,6 +164,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
                                                  int nb, int S) {
     static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
     constexpr int MX = N / GX, MY = N / GY;
+    // slots that can hold box points (all of them unless PR)
+    constexpr u64mask XB = PR ? box_slots(N, GX, MX, N / 4) : full_mask(MX);
+    constexpr u64mask YB = PR ? box_slots(N, GY, MY, N / 4) : full_mask(MY);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R> *twN = reinterpret_cast<cplx<R> *>(smem_raw);  // [N]
     cplx<R> *sub = twN + N;                                 // [P][N][S]
@@ -190,8 +200,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
             cplx<R> *col = sub + p * N * S + u;
             cplx<R> v[MY];
 #pragma unroll
-            for (int j = 0; j < MY; j++) v[j] = col[(L.g + GY * j) * S];
-            line_dif<+1, GY, MY>(v, L, twN);
+            for (int j = 0; j < MY; j++)
+                if (has_bit(YB, j)) v[j] = col[(L.g + GY * j) * S];
+            line_dif<+1, GY, MY, R, YB>(v, L, twN);
 #pragma unroll
             for (int k = 0; k < MY; k++) col[(k + MY * L.k2) * S] = v[k];
         }
@@ -209,18 +220,18 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
 #pragma unroll
             for (int j = 0; j < MX; j++) {
                 const int x = L.g + GX * j;
-                v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
+                if (has_bit(XB, j)) v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
             }
-            line_dif<+1, GX, MX>(v, L, twN);
+            line_dif<+1, GX, MX, R, XB>(v, L, twN);
             const R *vy = vr + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
 #pragma unroll
             for (int k = 0; k < MX; k++) v[k] = scale(v[k], vy[k]);
-            line_dit<-1, GX, MX>(v, L, twN);
+            line_dit<-1, GX, MX, R, XB>(v, L, twN);
             // gap columns go to the trash column nx + 1
 #pragma unroll
             for (int m = 0; m < MX; m++) {
                 const int x = L.g + GX * m;
-                row[x <= kd1 ? x : x >= xhi ? x - ushift : nx + 1] = v[m];
+                if (has_bit(XB, m)) row[x <= kd1 ? x : x >= xhi ? x - ushift : nx + 1] = v[m];
             }
         }
     }
@@ -237,9 +248,11 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
             cplx<R> v[MY];
 #pragma unroll
             for (int k = 0; k < MY; k++) v[k] = col[(k + MY * L.k2) * S];
-            line_dit<-1, GY, MY>(v, L, twN);
+            line_dit<-1, GY, MY, R, YB>(v, L, twN);
+            // rows outside the y box are not gathered back
 #pragma unroll
-            for (int m = 0; m < MY; m++) col[(L.g + GY * m) * S] = v[m];
+            for (int m = 0; m < MY; m++)
+                if (has_bit(YB, m)) col[(L.g + GY * m) * S] = v[m];
         }
     }
     __syncthreads();

@@ -55,11 +55,12 @@ EXPORTS = (
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
-TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN = 0, 1, 2, 3, 4, 5
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN, TUNE_COL = 0, 1, 2, 3, 4, 5, 6
 # plane-stage kernels (values of TUNE_PLANE): 0 LDS two-pass, 1-3 register
 # lines, 4-6 LDS two-pass exchanged within one wavefront, 7 register x rows
-# with wave-exchanged LDS y lines
-NPLANE_VAR = 8
+# with wave-exchanged LDS y lines, 8-11 register shapes that skip the slots
+# outside the sphere box (meshes with the box inside N/4)
+NPLANE_VAR = 12
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back", "k_proj_kb")
@@ -429,7 +430,7 @@ class Plan:
 
     def tune(self, psi_t, hpsi_t, prec=PREC_FP32, stream=None):
         """Autotune the plan's kernels on these device arrays (hpsi_t receives
-        H psi); returns the choice {"plane", "proj", "back", "splitk", "kin"}."""
+        H psi); returns the choice {"plane", "proj", "back", "splitk", "kin", "col"}."""
         import torch
 
         neig = psi_t.shape[0]
@@ -442,7 +443,8 @@ class Plan:
                 "proj": self.get_variant(prec, TUNE_PROJ),
                 "back": self.get_variant(prec, TUNE_BACK),
                 "splitk": self.get_variant(prec, TUNE_SPLITK),
-                "kin": self.get_variant(prec, TUNE_KIN)}
+                "kin": self.get_variant(prec, TUNE_KIN),
+                "col": self.get_variant(prec, TUNE_COL)}
 
     def set_potential(self, vscr):
         v = np.ascontiguousarray(vscr, np.float64)

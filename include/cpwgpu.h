@@ -201,7 +201,10 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *                       within one wavefront (256, 192, 128 threads per
  *                       block; where the plane fits in LDS), 7 = register
  *                       x rows of shape 1 with the y lines through LDS
- *                       exchanged within one wavefront; -1 = the default for
+ *                       exchanged within one wavefront, 8..11 = register
+ *                       shapes that neither load, transform nor store the
+ *                       register slots outside the sphere box (where the box
+ *                       lies inside |x|, |y| <= N/4); -1 = the default for
  *                       the mesh
  *   CPWGPU_TUNE_SPLITK  target number of blocks of the split-K projection
  *                       GEMM (64..4096); -1 = default (256)
@@ -220,13 +223,17 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *   CPWGPU_TUNE_SPIN    plane kernel of cpwgpu_plan_apply_spin with nsp = 4: 0 =
  *                       LDS two-pass lines, 1 = register-resident lines (built
  *                       when 8 divides the mesh length); -1 = default (1
- *                       where built, else 0) */
+ *                       where built, else 0)
+ *   CPWGPU_TUNE_COL     z-column kernels: 0 = LDS two-pass lines, 1 =
+ *                       register-resident lines (built where a column fits in
+ *                       the registers of 8 or 4 lanes); -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2
 #define CPWGPU_TUNE_BACK 3
 #define CPWGPU_TUNE_KIN 4
 #define CPWGPU_TUNE_SPIN 5
+#define CPWGPU_TUNE_COL 6
 int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
 /* Current choice (the value used by the next apply, defaults resolved). */
 int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);

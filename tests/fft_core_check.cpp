@@ -94,6 +94,37 @@ template <int N, int DIR> double check_two_pass() {
     return e1 > e2 ? e1 : e2;
 }
 
+// pruned transform: inputs outside IN hold NaN (must never be read), only the
+// outputs in OUT are compared with the DFT of the input with zeros outside IN
+template <int M, int DIR, u64mask IN, u64mask OUT> double check_pruned() {
+    auto x = rnd(M);
+    for (int j = 0; j < M; j++)
+        if (!has_bit(IN, j)) x[j] = mk<double>(0.0, 0.0);
+    C v[M];
+    for (int j = 0; j < M; j++) v[j] = has_bit(IN, j) ? x[j] : mk<double>(NAN, NAN);
+    DftP<M, DIR, double, IN, OUT>::run(v);
+    auto y = naive(x, DIR);
+    std::vector<C> got, want;
+    for (int m = 0; m < M; m++)
+        if (has_bit(OUT, m)) {
+            got.push_back(v[m]);
+            want.push_back(y[m]);
+        }
+    const double e = relerr(got, want);
+    return e == e ? e : 1.0;  // NaN read -> failure
+}
+
+// the masks of the register line transforms: slots of a length-N line on G
+// lanes whose x can fall in the box |x| <= N/4
+template <int N, int G, int DIR> double check_line_masks() {
+    constexpr int M = N / G;
+    constexpr u64mask B = box_slots(N, G, M, N / 4);
+    const double a = check_pruned<M, DIR, B, full_mask(M)>();
+    const double b = check_pruned<M, DIR, full_mask(M), B>();
+    const double c = check_pruned<M, DIR, B, B>();
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
 int main() {
     double e = 0;
     auto up = [&](double v, const char *what, int m) {
@@ -104,6 +135,16 @@ int main() {
     D(2) D(3) D(4) D(5) D(6) D(8) D(9) D(10) D(12) D(15) D(16) D(18) D(20)
 #define T(N) up(check_two_pass<N, -1>(), "two_pass", N); up(check_two_pass<N, 1>(), "two_pass", N);
     T(24) T(32) T(36) T(40) T(48) T(54) T(60) T(64) T(72) T(80) T(96) T(100) T(108) T(120) T(128) T(144)
+#define PM(N, G) up(check_line_masks<N, G, -1>(), "pruned", N * 100 + G); \
+    up(check_line_masks<N, G, 1>(), "pruned", N * 100 + G);
+    PM(24, 2) PM(24, 4) PM(48, 2) PM(48, 4) PM(48, 8) PM(64, 2) PM(64, 4) PM(64, 8) PM(72, 2) PM(72, 4)
+    PM(72, 8) PM(80, 2) PM(80, 4) PM(80, 8) PM(96, 2) PM(96, 4) PM(96, 8) PM(108, 2) PM(108, 4) PM(128, 2)
+    PM(128, 4) PM(128, 8)
+    // irregular masks
+    up(check_pruned<9, 1, 0x1C7ull, 0x1FFull>(), "pruned", 9);
+    up(check_pruned<12, -1, 0x00Full, 0xF0Full>(), "pruned", 12);
+    up(check_pruned<36, 1, 0xE000003FFull, 0x3Full>(), "pruned", 36);
+    up(check_pruned<20, -1, 0x1ull, 0xFFFFFull>(), "pruned", 20);
     printf("max_rel_err %.3e\n", e);
     return e < 1e-14 ? 0 : 1;
 }

@@ -1,5 +1,6 @@
-"""Every plane-stage kernel variant and split-K target of the autotuner gives
-the reference's H|psi>, and the tuner's choice and cache behave.
+"""Every plane-stage and column kernel variant and split-K target of the
+autotuner gives the reference's H|psi>, and the tuner's choice and cache
+behave.
 
 FP64: relative 2-norm error <= 1e-12 against the reference's golden vectors;
 variants agree with each other to 1e-13 on all 128 bands of the 64-atom cell.
@@ -53,6 +54,20 @@ def _variants(hk, plan, prec):
     return out
 
 
+def _col_variants(hk, plan, prec):
+    out = []
+    for v in (0, 1):
+        try:
+            plan.set_variant(prec, hk.TUNE_COL, v)
+        except hk.CpwGpuError as e:
+            assert e.code == hk.ERR_MODE
+            continue
+        assert plan.get_variant(prec, hk.TUNE_COL) == v
+        out.append(v)
+    plan.set_variant(prec, hk.TUNE_COL, -1)
+    return out
+
+
 def _apply(torch, plan, pt, prec):
     ht = torch.zeros_like(pt)
     plan.apply(pt, ht, prec)
@@ -72,6 +87,9 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
         assert 0 in vs, vs
         if name in SPECIALISED:
             assert len(vs) >= 2, vs
+        cvs = _col_variants(hk, plan, prec)
+        if name in SPECIALISED:
+            assert cvs == [0, 1], cvs
         for v in vs:
             plan.set_variant(prec, hk.TUNE_PLANE, v)
             assert plan.get_variant(prec, hk.TUNE_PLANE) == v
@@ -79,8 +97,10 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
                 plan.set_variant(prec, hk.TUNE_PROJ, jv)
                 plan.set_variant(prec, hk.TUNE_BACK, min(jv, 3))
                 plan.set_variant(prec, hk.TUNE_KIN, jv)  # kinetic term in the epilogue / the gather
+                plan.set_variant(prec, hk.TUNE_COL, cvs[jv % len(cvs)])
                 h = _apply(torch, plan, pt, prec).cpu().numpy().T
                 assert rel(h, href) < tol, (name, prec, v, jv, rel(h, href))
+        plan.set_variant(prec, hk.TUNE_COL, -1)
     plan.close()
 
 
@@ -108,6 +128,11 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "kin", kv)
         plan.set_variant(prec, hk.TUNE_KIN, -1)
+        for cv in _col_variants(hk, plan, prec):
+            plan.set_variant(prec, hk.TUNE_COL, cv)
+            h = _apply(torch, plan, pt, prec)
+            assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "col", cv)
+        plan.set_variant(prec, hk.TUNE_COL, -1)
         for jv in (0, 1, 2, 3, 4, 5):
             plan.set_variant(prec, hk.TUNE_PROJ, jv)
             for sk in (64, 512, 4096):
@@ -124,7 +149,8 @@ def test_bad_variants_are_rejected(hk):
     d = synth.make_config(0, nbands=2)
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, hk.NPLANE_VAR), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
-                        (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (hk.TUNE_SPIN, 2), (7, 0)):
+                        (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (hk.TUNE_SPIN, 2),
+                        (hk.TUNE_COL, 2), (7, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -135,6 +161,10 @@ def test_bad_variants_are_rejected(hk):
     with pytest.raises(hk.CpwGpuError):
         pg.set_variant(hk.PREC_FP32, hk.TUNE_SPIN, 1)
     assert pg.get_variant(hk.PREC_FP32, hk.TUNE_SPIN) == 0
+    # so do the register column kernels
+    with pytest.raises(hk.CpwGpuError):
+        pg.set_variant(hk.PREC_FP64, hk.TUNE_COL, 1)
+    assert pg.get_variant(hk.PREC_FP64, hk.TUNE_COL) == 0
     pg.close()
     plan.close()
 
@@ -155,7 +185,8 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         assert choice["plane"] in _variants(hk, plan, prec)
         assert choice["proj"] in range(6) and choice["back"] in range(5)
         assert choice["splitk"] in (128, 256, 512, 1024)
-        assert choice["kin"] in (0, 1)
+        assert choice["kin"] in (0, 1) and choice["col"] in (0, 1)
+        plan.set_variant(prec, hk.TUNE_COL, choice["col"])
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
         plan.set_variant(prec, hk.TUNE_PROJ, choice["proj"])
         plan.set_variant(prec, hk.TUNE_BACK, choice["back"])
@@ -173,7 +204,8 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
                   "proj": plan.get_variant(hk.PREC_FP32, hk.TUNE_PROJ),
                   "back": plan.get_variant(hk.PREC_FP32, hk.TUNE_BACK),
                   "splitk": plan.get_variant(hk.PREC_FP32, hk.TUNE_SPLITK),
-                  "kin": plan.get_variant(hk.PREC_FP32, hk.TUNE_KIN)}
+                  "kin": plan.get_variant(hk.PREC_FP32, hk.TUNE_KIN),
+                  "col": plan.get_variant(hk.PREC_FP32, hk.TUNE_COL)}
     assert len([l for l in cache.read_text().split("\n") if l.strip()]) == 2
     plan.close()
     plan2.close()
