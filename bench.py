@@ -322,10 +322,25 @@ def bench_ditsp(torch, hk, plan, d, nb, np):
     lowest nb bands to max residual^2 < epspsi^2 (epspsi = 1e-6), FP64 and
     FP32-mixed; wall time of the whole call.  "cold": from the small-matrix
     start.  "scf_step": from the converged bands of a potential 2% weaker,
-    as in a self-consistency step."""
+    as in a self-consistency step.  Default iteration (Chebyshev-filtered
+    subspace iteration); "block_fp64_cold": the locally optimal block
+    iteration on the same cold start, for comparison."""
     mtxd = d["mtxd"]
     out = {}
     eis = {}
+    # the default iteration (Chebyshev-filtered), then the block iteration on the same start
+    hk.set_ditsp_method(hk.DITSP_BLOCK)
+    psi = torch.zeros((nb, mtxd), dtype=torch.complex128, device="cuda")
+    hpsi = torch.zeros_like(psi)
+    plan.ditsp(psi, hpsi, prec=hk.PREC_FP64, icmax=2, epspsi=1e-6)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eib, ifail, st = plan.ditsp(psi, hpsi, prec=hk.PREC_FP64, icmax=200, epspsi=1e-6)
+    dt = time.perf_counter() - t0
+    out["block_fp64_cold"] = {"seconds": round(dt, 4), "ifail": ifail, **st}
+    log(f"bench: ditsp block iteration fp64 cold: {dt * 1e3:.1f} ms, {st['iters']} iterations, ifail {ifail}")
+    hk.set_ditsp_method(hk.DITSP_CHEB)
+    del psi, hpsi
     for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
         psi = torch.zeros((nb, mtxd), dtype=torch.complex128, device="cuda")
         hpsi = torch.zeros_like(psi)
@@ -352,6 +367,8 @@ def bench_ditsp(torch, hk, plan, d, nb, np):
             f"{st['hpsi_products']} H|psi> products, ifail {ifail}")
         del psi, hpsi
     out["eig_fp32_vs_fp64_ha"] = float(np.abs(eis["fp64"] - eis["fp32"]).max())
+    out["eig_vs_block_ha"] = float(np.abs(eis["fp64"] - eib).max())
+    out["method"] = "Chebyshev-filtered subspace iteration"
     out["bands"] = nb
     out["epspsi"] = 1e-6
     try:

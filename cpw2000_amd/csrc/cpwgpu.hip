@@ -40,6 +40,7 @@ thread_local std::string g_err;
 std::mutex g_mu;
 int g_device = 0;
 int g_prec = CPWGPU_PREC_FP64;
+int g_ditsp_method = CPWGPU_DITSP_CHEB;
 int g_overlap = 1;
 hipStream_t g_stream = nullptr;
 
@@ -1744,6 +1745,156 @@ int size_mtxds(const std::vector<double> &hd, int neig) {
     return mtxd;
 }
 
+// Largest eigenvalue of the symmetric tridiagonal (a, b) (b[j] couples j, j+1):
+// bisection on the Sturm count in the Gershgorin interval
+double tridiag_max(const std::vector<double> &a, const std::vector<double> &b) {
+    const int k = (int)a.size();
+    double lo = HUGE_VAL, hi = -HUGE_VAL;
+    for (int j = 0; j < k; j++) {
+        const double r = (j > 0 ? std::fabs(b[j - 1]) : 0.0) + (j + 1 < k ? std::fabs(b[j]) : 0.0);
+        lo = std::min(lo, a[j] - r);
+        hi = std::max(hi, a[j] + r);
+    }
+    // number of eigenvalues below x
+    auto below = [&](double x) {
+        int n = 0;
+        double d = 1.0;
+        for (int j = 0; j < k; j++) {
+            d = a[j] - x - (j > 0 ? b[j - 1] * b[j - 1] / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0) n++;
+        }
+        return n;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-12 * std::max(1.0, std::fabs(hi)); it++) {
+        const double mid = 0.5 * (lo + hi);
+        if (below(mid) == k) hi = mid;
+        else lo = mid;
+    }
+    return hi;
+}
+
+// Upper bound of the spectrum of H: DT_LANCZOS Lanczos steps (FP64 H) from a
+// random vector; the largest Ritz value plus the norm of the last residual.
+// buf: three scratch columns (ld mtxd).
+constexpr int DT_LANCZOS = 12;
+template <typename HA>
+int lanczos_upper(HA &happly, cpwgpu_plan_s *p, int mtxd, double2 *buf, hipStream_t st, double *bup) {
+    int rc;
+    double2 *v = buf, *w = buf + mtxd, *vp = buf + 2 * (size_t)mtxd;
+    double2 *dot = (double2 *)g_dt.m.p;
+    hipLaunchKernelGGL(k_random_cols, dt_grid(mtxd, 1), dim3(DT_THREADS), 0, st, v, (long)mtxd, mtxd, 0, 1,
+                       (const double *)p->ekpg.p, DT_SEED ^ 0x9E3779B97F4A7C15ull);
+    hipLaunchKernelGGL(k_normalize_cols, dim3(1), dim3(DT_THREADS), 0, st, v, (long)mtxd, mtxd);
+    std::vector<double> al, be;
+    double bprev = 0.0;
+    for (int j = 0; j < std::min(DT_LANCZOS, mtxd); j++) {
+        if ((rc = happly(CPWGPU_PREC_FP64, v, w, 1))) return rc;
+        double2 h;
+        if ((rc = gram(v, mtxd, 1, w, mtxd, 1, mtxd, nullptr, dot, st))) return rc;
+        HIPCHK(hipMemcpyAsync(&h, dot, sizeof h, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const double a = h.x;
+        // w -= a v + bprev vp
+        hipLaunchKernelGGL(k_cheb, dt_grid(mtxd, 1), dim3(DT_THREADS), 0, st, w, (const double2 *)w,
+                           (const double2 *)v, (const double2 *)vp, (long)mtxd, mtxd, a, 1.0, -bprev);
+        if ((rc = gram(w, mtxd, 1, w, mtxd, 1, mtxd, nullptr, dot, st))) return rc;
+        HIPCHK(hipMemcpyAsync(&h, dot, sizeof h, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const double b = std::sqrt(std::max(h.x, 0.0));
+        al.push_back(a);
+        if (b <= 1e-12 * std::fabs(a) || j + 1 == std::min(DT_LANCZOS, mtxd)) {
+            be.push_back(b);
+            break;
+        }
+        be.push_back(b);
+        // vp <- w / b, then (v, vp) swap roles
+        hipLaunchKernelGGL(k_cheb, dt_grid(mtxd, 1), dim3(DT_THREADS), 0, st, vp, (const double2 *)w,
+                           (const double2 *)w, (const double2 *)w, (long)mtxd, mtxd, 0.0, 1.0 / b, 0.0);
+        std::swap(v, vp);
+        bprev = b;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> off(be.begin(), be.end() - 1);
+    *bup = tridiag_max(al, off) + be.back();
+    return CPWGPU_OK;
+}
+
+// Degree of the Chebyshev filter.  The filter grows the lowest Ritz value's
+// direction by T_m(|l0|) (l0 = its image in the mapped interval) against at
+// most 1 for the damped part; the degree keeps that ratio near DT_CHEB_AMP, so
+// the Gram matrix of the filtered block stays well conditioned for the
+// Cholesky-QR.  CPWGPU_DITSP_DEGREE (4..40) fixes it instead.
+constexpr double DT_CHEB_AMP = 1e4;
+int cheb_degree(double l0) {
+    const char *env = getenv("CPWGPU_DITSP_DEGREE");
+    const int fixed = env && *env ? atoi(env) : 0;
+    if (fixed >= 4 && fixed <= 40) return fixed;
+    const double x = std::max(std::fabs(l0), 1.0 + 1e-9);
+    const int m = (int)(std::acosh(DT_CHEB_AMP) / std::acosh(x));
+    return std::min(24, std::max(4, m));
+}
+
+// One step of Chebyshev-filtered subspace iteration (Zhou and Saad, the
+// scaled three-term recurrence): the nb vectors X (HX = H X, Ritz values E,
+// host copy wv) are filtered by the degree-m Chebyshev polynomial that damps
+// [wv[nb-1], bup] and grows below it, orthonormalised (Cholesky-QR, columns
+// lost to dependence replaced by random ones) and rotated to the Ritz vectors
+// of H in their span: X, HX, E on return.  Work space: columns nb..3nb of
+// X and HX, and XB, HXB.
+template <typename HA>
+int cheb_filter_step(HA &happly, cpwgpu_plan_s *p, int pc, int mtxd, int nb, int neig, double2 *X, double2 *HX,
+                     double2 *XB, double2 *HXB, double *E, const std::vector<double> &wv, double &bup,
+                     hipStream_t st) {
+    int rc;
+    const size_t col = 16 * (size_t)mtxd;
+    const double a0 = wv[0], a = wv[nb - 1];
+    // a spectrum bound below the block's top Ritz value cannot be right: widen it
+    if (!(bup > a + 1e-6 * std::max(1.0, std::fabs(a)))) bup = a + std::max(1.0, a - a0);
+    const double e = 0.5 * (bup - a), c = 0.5 * (bup + a);
+    double sigma = e / (a0 - c);
+    const double tau = 2.0 / sigma;
+    const int deg = cheb_degree((a0 - c) / e);
+    double2 *B0 = X, *B1 = X + (size_t)nb * mtxd, *B2 = X + 2 * (size_t)nb * mtxd;
+    double2 *HB = HX + (size_t)nb * mtxd;
+    const dim3 grid = dt_grid(mtxd, nb);
+    // y1 = (H x - c x) sigma / e
+    hipLaunchKernelGGL(k_cheb, grid, dim3(DT_THREADS), 0, st, B1, (const double2 *)HX, (const double2 *)B0,
+                       (const double2 *)B0, (long)mtxd, mtxd, c, sigma / e, 0.0);
+    for (int i = 2; i <= deg; i++) {
+        if ((rc = happly(pc, B1, HB, nb))) return rc;
+        const double s1 = 1.0 / (tau - sigma);
+        hipLaunchKernelGGL(k_cheb, grid, dim3(DT_THREADS), 0, st, B2, (const double2 *)HB, (const double2 *)B1,
+                           (const double2 *)B0, (long)mtxd, mtxd, c, 2.0 * s1 / e, -sigma * s1);
+        double2 *t = B0;
+        B0 = B1;
+        B1 = B2;
+        B2 = t;
+        sigma = s1;
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_normalize_cols, dim3(nb), dim3(DT_THREADS), 0, st, B1, (long)mtxd, mtxd);
+    int kept = 0;
+    if ((rc = orthonormalize(mtxd, nullptr, mtxd, 0, B1, mtxd, nb, &kept, st))) return rc;
+    if (kept < nb) {
+        hipLaunchKernelGGL(k_random_cols, dt_grid(mtxd, nb - kept), dim3(DT_THREADS), 0, st, B1, (long)mtxd,
+                           mtxd, kept, nb - kept, (const double *)p->ekpg.p, DT_SEED + (unsigned long long)kept);
+        hipLaunchKernelGGL(k_normalize_cols, dim3(nb - kept), dim3(DT_THREADS), 0, st, B1 + (size_t)kept * mtxd,
+                           (long)mtxd, mtxd);
+        int k2 = 0;
+        if ((rc = orthonormalize(mtxd, B1, mtxd, kept, B1 + (size_t)kept * mtxd, mtxd, nb - kept, &k2, st)))
+            return rc;
+        if (k2 < nb - kept) return fail(CPWGPU_ERR_NOTPD, "filtered vectors are linearly dependent");
+    }
+    if ((rc = happly(pc, B1, HB, nb))) return rc;
+    if ((rc = ritz_step(mtxd, B1, HB, nb, nb, XB, HXB, nullptr, st))) return rc;
+    HIPCHK(hipMemcpyAsync(X, XB, col * nb, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(HX, HXB, col * nb, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(E, g_dt.w.p, 8 * (size_t)nb, hipMemcpyDeviceToDevice, st));
+    (void)neig;
+    return CPWGPU_OK;
+}
+
 int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi, int neig, int mtxds,
                const double2 *hamsm, long ldhs, const double *hdiag_in, double2 *psi, long ldp,
                double2 *hpsi, long ldh, double *ei, int *ifail, int *stats, hipStream_t st) {
@@ -1910,6 +2061,11 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
     actp.reserve(nb);
     double xmax = 0.0, xprev = HUGE_VAL;
     bool conv = false;
+    // Chebyshev-filtered subspace iteration: an upper bound of the spectrum
+    // from a few Lanczos steps (FP64 H on one vector)
+    const bool cheb = g_ditsp_method == CPWGPU_DITSP_CHEB;
+    double bup = 0.0;
+    if (cheb && (rc = lanczos_upper(happly, p, mtxd, XB, st, &bup))) return rc;
     for (int ic = 0;; ic++) {
         const int nu = nb - nl;
         // residuals of the unlocked vectors; the locked ones keep theirs
@@ -1946,6 +2102,11 @@ int plan_ditsp(cpwgpu_plan_s *p, int prec, int icmax, int iguess, double epspsi,
             break;
         }
         if (ic >= icmax) break;
+        if (cheb) {
+            if ((rc = cheb_filter_step(happly, p, pc, mtxd, nb, neig, X, HX, XB, HXB, E, wv, bup, st))) return rc;
+            xprev = xmax;
+            continue;
+        }
         const double thr = 0.01 * (refined ? eps2 : sw2);
         // lock the lowest unlocked vectors that have converged
         while (nl < nb - 1 && res[nl] <= thr) nl++;
@@ -2612,6 +2773,13 @@ int cpwgpu_finalize(void) {
 }
 
 const char *cpwgpu_last_error(void) { return g_err.c_str(); }
+
+int cpwgpu_set_ditsp_method(int method) {
+    if (method != CPWGPU_DITSP_BLOCK && method != CPWGPU_DITSP_CHEB)
+        return fail(CPWGPU_ERR_MODE, "diagonalisation method %d", method);
+    g_ditsp_method = method;
+    return CPWGPU_OK;
+}
 
 int cpwgpu_set_overlap(int on) {
     g_overlap = on ? 1 : 0;

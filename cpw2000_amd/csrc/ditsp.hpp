@@ -146,4 +146,24 @@ __global__ __launch_bounds__(DT_THREADS) void k_fill(double *__restrict__ d, int
     if (k < n) d[k] = v;
 }
 
+// One step of the three-term Chebyshev recurrence on n columns (ld ldy):
+//   y2 = alpha (hy1 - c y1) + beta y0
+// elementwise, so y2 may be hy1 (the Lanczos update w -= a v + b v_prev uses
+// it with alpha = 1, c = a, beta = -b).
+__global__ __launch_bounds__(DT_THREADS) void k_cheb(double2 *y2, const double2 *hy1, const double2 *y1,
+                                                     const double2 *y0, long ldy, int mtxd, double c,
+                                                     double alpha, double beta) {
+    const int i = blockIdx.x * DT_THREADS + threadIdx.x, j = blockIdx.y;
+    if (i >= mtxd) return;
+    const long o = (long)j * ldy + i;
+    const double2 h = hy1[o], a = y1[o];
+    double2 r = make_double2(alpha * (h.x - c * a.x), alpha * (h.y - c * a.y));
+    if (beta != 0.0) {
+        const double2 b = y0[o];
+        r.x += beta * b.x;
+        r.y += beta * b.y;
+    }
+    y2[o] = r;
+}
+
 }  // namespace cpwgpu

@@ -41,7 +41,7 @@ PREC_FP32 = 1
 # every symbol declared in include/cpwgpu.h
 EXPORTS = (
     "cpwgpu_init", "cpwgpu_finalize", "cpwgpu_last_error", "cpwgpu_set_precision",
-    "cpwgpu_set_overlap",
+    "cpwgpu_set_overlap", "cpwgpu_set_ditsp_method",
     "cpwgpu_hk_psi_c16", "cpwgpu_plan_create", "cpwgpu_plan_destroy",
     "cpwgpu_plan_set_potential", "cpwgpu_plan_set_projectors",
     "cpwgpu_plan_set_projectors_device", "cpwgpu_plan_apply", "cpwgpu_plan_info",
@@ -141,6 +141,16 @@ def _iref(v):
 
 def set_precision(prec):
     _check(lib().cpwgpu_set_precision(int(prec)))
+
+
+DITSP_BLOCK, DITSP_CHEB = 0, 1
+
+
+def set_ditsp_method(method):
+    """Iteration of Plan.ditsp / ditsp_c16: DITSP_CHEB (Chebyshev-filtered
+    subspace iteration, default) or DITSP_BLOCK (locally optimal block
+    iteration)."""
+    _check(lib().cpwgpu_set_ditsp_method(int(method)))
 
 
 def set_overlap(on):

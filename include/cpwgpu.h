@@ -60,6 +60,16 @@ int cpwgpu_set_precision(int prec);
  * kernels (fork/join with events on the caller's stream); 0: every kernel on
  * the caller's stream, one after the other.  Same work and result. */
 int cpwgpu_set_overlap(int on);
+/* Iteration of cpwgpu_plan_ditsp / cpwgpu_ditsp_c16 (no counterpart in the
+ * reference, whose Src/ditsp/ditsp_c16_loop.f90 relaxes band by band):
+ * CPWGPU_DITSP_CHEB (default) = Chebyshev-filtered subspace iteration (the
+ * filter degree follows the spectrum bounds, 4..24; CPWGPU_DITSP_DEGREE in
+ * the environment fixes it); CPWGPU_DITSP_BLOCK = locally optimal block
+ * preconditioned iteration with locking.  Same start, convergence test and
+ * outputs. */
+#define CPWGPU_DITSP_BLOCK 0
+#define CPWGPU_DITSP_CHEB 1
+int cpwgpu_set_ditsp_method(int method);
 
 /* ---- drop-in ------------------------------------------------------------ */
 

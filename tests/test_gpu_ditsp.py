@@ -8,7 +8,9 @@ after its FP64 phase; the direct path (mtxds >= mtxd) to 1e-11 Ha.  Vectors
 orthonormal to 1e-12, hpsi equal to an independent FP64 H psi to 1e-12,
 every residual below epspsi.
 64-atom cell (72^3, 32 bands): FP64 and FP32-mixed eigenvalues agree to
-1e-9 Ha, residuals below epspsi with hpsi checked by an independent apply."""
+1e-9 Ha, residuals below epspsi with hpsi checked by an independent apply.
+Every test runs with both iterations: Chebyshev-filtered subspace iteration
+(the default) and the locally optimal block iteration."""
 
 import numpy as np
 import pytest
@@ -33,6 +35,13 @@ def hk():
     from cpw2000_amd import hk_psi
 
     return hk_psi
+
+
+@pytest.fixture(autouse=True, params=["cheb", "block"])
+def method(request, hk):
+    hk.set_ditsp_method(hk.DITSP_CHEB if request.param == "cheb" else hk.DITSP_BLOCK)
+    yield request.param
+    hk.set_ditsp_method(hk.DITSP_CHEB)
 
 
 @pytest.fixture(scope="module")
