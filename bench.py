@@ -283,7 +283,7 @@ def bench_kpoints(torch, hk, synth, np, steps):
         for p in plans[1:]:
             for what, key in ((hk.TUNE_PLANE, "plane"), (hk.TUNE_PROJ, "proj"),
                               (hk.TUNE_BACK, "back"), (hk.TUNE_SPLITK, "splitk"), (hk.TUNE_KIN, "kin"),
-                              (hk.TUNE_COL, "col")):
+                              (hk.TUNE_COL, "col"), (hk.TUNE_SCHED, "sched")):
                 p.set_variant(pid, what, c[key])
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)

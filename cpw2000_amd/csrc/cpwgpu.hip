@@ -192,6 +192,10 @@ struct cpwgpu_plan_s {
     int plane_var[2] = {-1, -1};
     // column kernels: 0 LDS lines, 1 register lines; -1 default
     int col_var[2] = {-1, -1};
+    // stream schedule of the two-stream apply: 0 (default) the whole
+    // non-local term beside the local one, the gather adding to it; 1 the
+    // projection beside the local term, the back-projection after the gather
+    int sched_var[2] = {-1, -1};
     // split-K target blocks of the projection GEMM per precision (-1 = 256)
     int splitk[2] = {-1, -1};
     // projection GEMM kernel per precision: 0 LDS-staged k_proj, 1 k_proj_direct
@@ -787,9 +791,13 @@ template <typename R> void splitk_shape(const cpwgpu_plan_s *p, int rows, int na
 
 // hpsi = beta*hpsi + sum_a beta_a xnlkb_a <beta_a|psi> (Src/cpw_lib/hk_psi_nl_c16.f90:82-96),
 // plus ekpg*psi when kin (the kinetic term, only for the plan's own rows)
+// phase: NL_ALL, or NL_PROJ (projection and D scaling only: D<beta|psi> left
+// in the workspace) then NL_BACK (the back-projection from it) on the same
+// workspace, possibly on another stream ordered after the first.
+enum NlPhase { NL_ALL, NL_PROJ, NL_BACK };
 template <typename R>
 int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi, long ldp, double *hpsi,
-             long ldh, int beta, bool kin, hipStream_t s, const Work *w = nullptr) {
+             long ldh, int beta, bool kin, hipStream_t s, const Work *w = nullptr, int phase = NL_ALL) {
     const size_t cs = sizeof(cplx<R>);
     const int nanl = pv_.nanl;
     const int rows = pv_.rows;
@@ -834,7 +842,7 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
         part = (cplx<R> *)p->part.p;
         dhd = (cplx<R> *)p->dhd.p;
     }
-    {
+    if (phase != NL_BACK) {
         ProfScope ps_(CPWGPU_K_PROJ, s);
         const dim3 grid(tm, tn, nsl);
         const double2 *ps = (const double2 *)psi;
@@ -855,11 +863,11 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
             hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
                                (const double2 *)psi, ldp, pt, rows, nanl, neig, kslice, nullptr);
     }
-    {
+    if (phase != NL_BACK) {
         ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
         launch_proj_reduce<R>((const cplx<R> *)part, pv_.xnl, dhd, nanl, neig, nsl, s);
     }
-    {
+    if (phase != NL_PROJ) {
         ProfScope ps_(CPWGPU_K_BACK, s);
         const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_VAR - 1 ? p->back_var[ix] : 0;
         const cplx<R> *dh = dhd;
@@ -909,7 +917,8 @@ int ensure_aux() {
 // hpsi; join (may be null): event the first gather waits for (that term).
 template <typename R>
 int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
-               bool acc, hipEvent_t join, hipStream_t s, bool kin_gather = false, const Work *w = nullptr) {
+               bool acc, hipEvent_t join, hipStream_t s, bool kin_gather = false, const Work *w = nullptr,
+               bool write_nokin = false) {
     const FftGeom &g = p->g;
     const size_t cs = sizeof(cplx<R>);
     const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
@@ -931,9 +940,12 @@ int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
         const int nb = std::min(nbc, neig - b0);
         // with the non-local term the back-projection already wrote it and the
         // kinetic term into hpsi; without it the gather adds the kinetic term
+        // write_nokin: the gather writes local/N (+ ekpg psi if kin_gather)
+        // and the back-projection adds the rest afterwards
+        const int mode = acc ? (COLF_ACC | (kin_gather ? COLF_KIN : 0))
+                             : write_nokin ? (kin_gather ? COLF_KIN : 0) : COLF_KIN;
         const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
-                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb,
-                           acc ? (COLF_ACC | (kin_gather ? COLF_KIN : 0)) : COLF_KIN};
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, mode};
         if (g.ncol > 0) {
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, s))) return st;
             if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, s))) return st;
@@ -961,6 +973,20 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
         // serial: non-local term first (writes hpsi), then the local term adds
         if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, s, w))) return st;
         return local_part<R>(p, neig, psi, ldp, hpsi, ldh, true, nullptr, s, kg, w);
+    }
+    if (nl && p->sched_var[sizeof(R) == 4 ? 1 : 0] == 1) {
+        // fork: projection and D scaling on the second stream, beside the
+        // column and plane kernels; the gather writes the local term, then the
+        // back-projection adds the non-local one on the caller's stream
+        if ((st = ensure_aux())) return st;
+        HIPCHK(hipEventRecord(g_fork, s));
+        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 1, !kg, g_aux, w, NL_PROJ)))
+            return st;
+        HIPCHK(hipEventRecord(g_join, g_aux));
+        if ((st = local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s, kg, w, true))) return st;
+        HIPCHK(hipStreamWaitEvent(s, g_join, 0));
+        return nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 1, !kg, s, w, NL_BACK);
     }
     if (nl) {
         // fork: non-local term on the second stream, written into hpsi
@@ -2429,9 +2455,10 @@ struct TuneKey {
     }
 };
 
-// one line per key: prec n1 n2 n3 nx ncol nanl neig | plane proj back splitk kin col
-// (lines without the last fields, kin and col, read them as -1)
-bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk, int *kin, int *col) {
+// one line per key: prec n1 n2 n3 nx ncol nanl neig | plane proj back splitk kin col sched
+// (lines without the last fields, kin, col and sched, read them as -1)
+bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *splitk, int *kin, int *col,
+                       int *sched) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return false;
     FILE *f = std::fopen(path, "r");
@@ -2440,9 +2467,9 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
     bool found = false;
     while (std::fgets(line, sizeof line, f)) {
         TuneKey r;
-        int pv, jv, bv, sk, kv = -1, cv = -1;
-        const int n = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2,
-                                  &r.n3, &r.nx, &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk, &kv, &cv);
+        int pv, jv, bv, sk, kv = -1, cv = -1, sv = -1;
+        const int n = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", &r.prec, &r.n1, &r.n2,
+                                  &r.n3, &r.nx, &r.ncol, &r.nanl, &r.neig, &pv, &jv, &bv, &sk, &kv, &cv, &sv);
         if (n < 12) continue;
         if (r == k) {  // the last entry for a key wins
             *plane = pv;
@@ -2451,6 +2478,7 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
             *splitk = sk;
             *kin = n >= 13 ? kv : -1;
             *col = n >= 14 ? cv : -1;
+            *sched = n >= 15 ? sv : -1;
             found = true;
         }
     }
@@ -2458,13 +2486,14 @@ bool tune_cache_lookup(const TuneKey &k, int *plane, int *proj, int *back, int *
     return found;
 }
 
-void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk, int kin, int col) {
+void tune_cache_store(const TuneKey &k, int plane, int proj, int back, int splitk, int kin, int col,
+                      int sched) {
     const char *path = std::getenv("CPWGPU_TUNE_CACHE");
     if (!path || !*path) return;
     FILE *f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
-                 k.nanl, k.neig, plane, proj, back, splitk, kin, col);
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", k.prec, k.n1, k.n2, k.n3, k.nx, k.ncol,
+                 k.nanl, k.neig, plane, proj, back, splitk, kin, col, sched);
     std::fclose(f);
 }
 
@@ -2500,8 +2529,9 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     const int ix = prec == CPWGPU_PREC_FP32 ? 1 : 0;
     const FftGeom &g = p->g;
     const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
-    int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1, cv = -1;
-    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv, &cv)) {
+    int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1, cv = -1, sv = -1;
+    if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv, &cv, &sv)) {
+        p->sched_var[ix] = sv == 0 || sv == 1 ? sv : -1;
         bool ok;
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
@@ -2578,9 +2608,24 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         }
     }
     p->kin_var[ix] = best_kv;
+    // stream schedule (two-stream applies only); with the back-projection
+    // after the gather it no longer shares the GPU: choose its kernel again
+    int best_sv = 0;
+    if (p->nanl > 0 && g_overlap) {
+        p->sched_var[ix] = 0;
+        if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &best))) return st;
+        p->sched_var[ix] = 1;
+        for (int bv = 0; bv < NBACK_VAR; bv++) {
+            p->back_var[ix] = bv;
+            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+            if (t < best) { best = t; best_sv = 1; best_bv = bv; }
+        }
+        p->back_var[ix] = best_bv;
+    }
+    p->sched_var[ix] = best_sv;
     // leave hpsi holding H psi of the chosen configuration
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
-    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv, best_cv);
+    tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv, best_cv, best_sv);
     return CPWGPU_OK;
 }
 
@@ -3165,6 +3210,11 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         plan->spin_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
+    if (what == CPWGPU_TUNE_SCHED) {
+        if (value > 1) return fail(CPWGPU_ERR_MODE, "stream schedule %d", value);
+        plan->sched_var[ix] = value < 0 ? -1 : value;
+        return CPWGPU_OK;
+    }
     if (what == CPWGPU_TUNE_COL) {
         bool ok = true;
         if (prec == CPWGPU_PREC_FP32) col_var_resolve<float>(plan, value, &ok);
@@ -3210,6 +3260,10 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
     }
     if (what == CPWGPU_TUNE_SPIN) {
         *value = prec == CPWGPU_PREC_FP32 ? spin_var_resolve<float>(plan) : spin_var_resolve<double>(plan);
+        return CPWGPU_OK;
+    }
+    if (what == CPWGPU_TUNE_SCHED) {
+        *value = plan->sched_var[ix] == 1 ? 1 : 0;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_COL) {

@@ -55,7 +55,7 @@ EXPORTS = (
 )
 
 # tunables of a plan (CPWGPU_TUNE_* in include/cpwgpu.h)
-TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN, TUNE_COL = 0, 1, 2, 3, 4, 5, 6
+TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN, TUNE_COL, TUNE_SCHED = range(8)
 # plane-stage kernels (values of TUNE_PLANE): 0 LDS two-pass, 1-3 register
 # lines, 4-6 LDS two-pass exchanged within one wavefront, 7 register x rows
 # with wave-exchanged LDS y lines, 8-11 register shapes that skip the slots
@@ -440,7 +440,8 @@ class Plan:
 
     def tune(self, psi_t, hpsi_t, prec=PREC_FP32, stream=None):
         """Autotune the plan's kernels on these device arrays (hpsi_t receives
-        H psi); returns the choice {"plane", "proj", "back", "splitk", "kin", "col"}."""
+        H psi); returns the choice {"plane", "proj", "back", "splitk", "kin", "col",
+        "sched"}."""
         import torch
 
         neig = psi_t.shape[0]
@@ -454,7 +455,8 @@ class Plan:
                 "back": self.get_variant(prec, TUNE_BACK),
                 "splitk": self.get_variant(prec, TUNE_SPLITK),
                 "kin": self.get_variant(prec, TUNE_KIN),
-                "col": self.get_variant(prec, TUNE_COL)}
+                "col": self.get_variant(prec, TUNE_COL),
+                "sched": self.get_variant(prec, TUNE_SCHED)}
 
     def set_potential(self, vscr):
         v = np.ascontiguousarray(vscr, np.float64)

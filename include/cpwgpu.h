@@ -236,7 +236,14 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *                       where built, else 0)
  *   CPWGPU_TUNE_COL     z-column kernels: 0 = LDS two-pass lines, 1 =
  *                       register-resident lines (built where a column fits in
- *                       the registers of 8 or 4 lanes); -1 = default (0) */
+ *                       the registers of 8 or 4 lanes); -1 = default (0)
+ *   CPWGPU_TUNE_SCHED   stream schedule with overlap on: 0 = the non-local
+ *                       term (projection, D scaling, back-projection) on the
+ *                       second stream beside the local term, the gather adding
+ *                       the local term to it; 1 = the projection and D scaling
+ *                       beside the local term, the gather writing H psi, then
+ *                       the back-projection adding to it on the caller's
+ *                       stream; -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2
@@ -244,6 +251,7 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
 #define CPWGPU_TUNE_KIN 4
 #define CPWGPU_TUNE_SPIN 5
 #define CPWGPU_TUNE_COL 6
+#define CPWGPU_TUNE_SCHED 7
 int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value);
 /* Current choice (the value used by the next apply, defaults resolved). */
 int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value);
