@@ -2623,6 +2623,19 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         p->back_var[ix] = best_bv;
     }
     p->sched_var[ix] = best_sv;
+    // the plane kernels once more, beside the chosen non-local kernels (how
+    // they share the GPU with the GEMM stream decides as much as their speed)
+    best = 3.4e38f;
+    for (int v = 0; v < NPLANE_VAR; v++) {
+        bool ok;
+        if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, v, &ok);
+        else plane_var_resolve<double>(p, v, &ok);
+        if (!ok) continue;
+        p->plane_var[ix] = v;
+        if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 8, &t))) return st;
+        if (t < best) { best = t; best_pv = v; }
+    }
+    p->plane_var[ix] = best_pv;
     // leave hpsi holding H psi of the chosen configuration
     if ((st = cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s))) return st;
     tune_cache_store(key, best_pv, best_jv, best_bv, best_sk, best_kv, best_cv, best_sv);

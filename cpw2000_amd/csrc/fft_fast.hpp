@@ -266,6 +266,14 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
         const int y = y0 + r;
         const bool act = r < RB && y < N2;
         cplx<R> v[B1 > A1 ? B1 : A1];
+        // V(r) of this thread's points, loaded before the first pass so that
+        // the global round trip overlaps it
+        R vv[A1];
+        if (act) {
+            const R *vy = vz + (long)y * g.id;
+#pragma unroll
+            for (int q = 0; q < A1; q++) vv[q] = vy[t + B1 * q];
+        }
         // inverse x transform, pass 1: x = t + A1 j, only box x are nonzero
         const cplx<R> *srow = sub + y * nxp;
         if (act && t < A1) {
@@ -282,9 +290,8 @@ __global__ __launch_bounds__(FT) void k_plane_fast(cplx<R> *__restrict__ colbuf,
 #pragma unroll
             for (int s = 0; s < A1; s++) v[s] = row[rpos<N1>(s + A1 * t)];
             dft<A1, +1, R>(v);
-            const R *vy = vz + (long)y * g.id;
 #pragma unroll
-            for (int q = 0; q < A1; q++) v[q] = scale(v[q], vy[t + B1 * q]);
+            for (int q = 0; q < A1; q++) v[q] = scale(v[q], vv[q]);
             dft<A1, -1, R>(v);
 #pragma unroll
             for (int m = 1; m < A1; m++) {

@@ -134,6 +134,14 @@ __global__ __launch_bounds__(T, W) void k_plane_wave(cplx<R> *__restrict__ colbu
         const bool act = t < B1 && y < N2;
         cplx<R> v[B1 > A1 ? B1 : A1];
         const cplx<R> *srow = sub + y * nxp;
+        // V(r) of this lane's points, loaded now so that the global round trip
+        // overlaps the first pass
+        R vv[A1];
+        if (act) {
+            const R *vy = vz + (long)y * g.id;
+#pragma unroll
+            for (int q = 0; q < A1; q++) vv[q] = vy[t + B1 * q];
+        }
         if (act && t < A1) {
             // inverse x transform, pass 1: x = t + A1 j, only box x are nonzero
 #pragma unroll
@@ -149,9 +157,8 @@ __global__ __launch_bounds__(T, W) void k_plane_wave(cplx<R> *__restrict__ colbu
 #pragma unroll
             for (int s = 0; s < A1; s++) v[s] = row[rpos<N1>(s + A1 * t)];
             dft<A1, +1, R>(v);
-            const R *vy = vz + (long)y * g.id;
 #pragma unroll
-            for (int q = 0; q < A1; q++) v[q] = scale(v[q], vy[t + B1 * q]);
+            for (int q = 0; q < A1; q++) v[q] = scale(v[q], vv[q]);
             dft<A1, -1, R>(v);
 #pragma unroll
             for (int m = 1; m < A1; m++) {
