@@ -116,11 +116,13 @@ def local_roofline(info, nb, prec, per):
 
 
 def measured_traffic(prec, kernel, variant, n1):
-    """HBM bytes per launch of a plane-stage or projector kernel measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by a copy of known size) on
-    this workload, recorded in profiles/roofline_traffic.json under
-    "<prec>/<plane|proj|back><variant>/<n1>"; None if that file has no entry."""
-    fam = {"k_plane": "plane", "k_proj": "proj", "k_back": "back"}.get(kernel)
+    """HBM bytes per launch of a plane-stage, column or projector kernel
+    measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by a copy of
+    known size) on this workload, recorded in profiles/roofline_traffic.json
+    under "<prec>/<plane|col_bwd|col_fwd|proj|back><variant>/<n1>"; None if
+    that file has no entry."""
+    fam = {"k_plane": "plane", "k_proj": "proj", "k_back": "back", "k_col_bwd": "col_bwd",
+           "k_col_fwd": "col_fwd"}.get(kernel)
     if fam is None:
         return None, None
     path = os.path.join(ROOT, "profiles", "roofline_traffic.json")
@@ -218,7 +220,8 @@ def cpu_baseline(d, seconds, nb=8):
 
 def projector_cpu_baseline(d, pp, seconds):
     """The reference proj_nl_kb_c16 (flang build, one core) when built, else the
-    numpy restatement, on the same inputs as the device build; bounded sample."""
+    numpy restatement, on the same inputs as the device build; bounded sample.
+    Beside it ("openmp"), the reference's OpenMP build on every host core."""
     import ctypes as C
 
     import numpy as np
@@ -247,17 +250,31 @@ def projector_cpu_baseline(d, pp, seconds):
         fn = lambda: PK.proj_nl_kb_c16(rk, m, isort, nanl, d["ng"], kgv, pp["nqnl"], pp["delqnl"], vkb,
                                        pp["nkb"], pp["ntype"], pp["natom"], pp["rat"], pp["adot"], m, nanl,
                                        kgv.shape[1])
-    fn()
-    t0 = time.perf_counter()
-    calls = 0
-    while True:
-        fn()
-        calls += 1
-        dt = time.perf_counter() - t0
-        if dt >= seconds:
-            break
-    return {"ms_per_call": round(dt / calls * 1e3, 2), "cores": 1, "kind": kind,
-            "sample": f"{calls} calls on the same inputs ({dt:.1f} s)"}
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        calls = 0
+        while True:
+            f()
+            calls += 1
+            dt = time.perf_counter() - t0
+            if dt >= seconds:
+                return dt, calls
+
+    dt, calls = timed(fn)
+    out = {"ms_per_call": round(dt / calls * 1e3, 2), "cores": 1, "kind": kind,
+           "sample": f"{calls} calls on the same inputs ({dt:.1f} s)"}
+    lomp = O.ref_omp_lib()
+    if kind == "reference" and lomp is not None:
+        fo = lambda: lomp.proj_nl_kb_c16_(
+            p(rk), i(m), p(isort), i(nanl), i(d["ng"]), p(kgv), p(pp["nqnl"]), p(pp["delqnl"]), p(vkb),
+            p(pp["nkb"]), i(pp["ntype"]), p(pp["natom"]), p(pp["rat"]), p(pp["adot"]), p(anl), p(xnl),
+            i(vkb.shape[3]), i(pp["rat"].shape[1]), i(vkb.shape[0] - 3), i(m), i(nanl), i(kgv.shape[1]))
+        dt, calls = timed(fo)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        out["openmp"] = {"ms_per_call": round(dt / calls * 1e3, 2), "cores": threads,
+                         "sample": f"{calls} calls ({dt:.1f} s)"}
+    return out
 
 
 def bench_kpoints(torch, hk, synth, np, steps):
@@ -621,7 +638,7 @@ def issue_bound(prec, kernel, variant, n1, avg_ms, rates):
 
 
 ISSUE_KINDS = ("v_fma_f64", "v_add_f64", "v_fma_f32", "v_add_u32", "v_mov_b32_dpp", "v_fma_f64+v_mov_b32_dpp",
-               "v_add_f32", "v_add_f32_dpp")
+               "v_add_f32", "v_add_f32_dpp", "v_permlane32_swap", "v_permlane16_swap")
 
 
 def issue_rates():
@@ -870,6 +887,12 @@ def main():
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},
     }
+    # measured HBM traffic of each kernel in its tuned variant, where recorded
+    fams = {"k_plane": "plane", "k_proj": "proj", "k_back": "back", "k_col_bwd": "col", "k_col_fwd": "col"}
+    for p in out["kernel_roofline"]:
+        for e in out["kernel_roofline"][p]:
+            if e["kernel"] in fams:
+                e["traffic"], _ = measured_traffic(p, e["kernel"], tuned[p][fams[e["kernel"]]], info["n1"])
     if args.calib_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         src.fill_(1.0)
@@ -898,7 +921,7 @@ def main():
     for key, prec in (("roofline", "fp32"), ("roofline_fp64", "fp64")):
         kern = out[key]["kernel"]
         fam = {"k_plane": "plane", "k_proj": "proj", "k_back": "back"}.get(kern)
-        if fam is not None:
+        if fam is not None and out[key].get("traffic") is None:
             tb, src = measured_traffic(prec, kern, tuned[prec][fam], info["n1"])
             out[key]["traffic"] = tb
             out[key]["traffic_source"] = src

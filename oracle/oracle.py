@@ -29,6 +29,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_SO = os.path.join(HERE, "liborc.so")
 REF_SO = os.path.join(HERE, "_ref", "libcpwref.so")
+# the projector construction alone, built with OpenMP (its all-core baseline)
+REF_OMP_SO = os.path.join(HERE, "_ref", "libcpwref_omp.so")
 
 _orc = None
 _ref = None
@@ -64,6 +66,17 @@ def orc_lib():
 
 def have_ref():
     return os.path.exists(REF_SO)
+
+
+_ref_omp = None
+
+
+def ref_omp_lib():
+    """The OpenMP build of the reference proj_nl_kb_c16, or None where not built."""
+    global _ref_omp
+    if _ref_omp is None and os.path.exists(REF_OMP_SO):
+        _ref_omp = C.CDLL(REF_OMP_SO)
+    return _ref_omp
 
 
 def ref_lib():
