@@ -41,6 +41,41 @@ template <> struct MF<float> {
 
 constexpr int GEMM_THREADS = 256;
 
+// Workgroups are dealt to the 8 XCDs round robin on their linear id.
+// xcd_linear renumbers the T blocks of a grid so that each XCD runs one
+// contiguous range of the new ids, a bijection on [0, T): with
+// per = ceil(T/8), XCDs 0 .. full-1 (full = T - 8 (per - 1)) take per ids
+// each, the others per - 1.
+// Tiles with neighbouring ids share operand rows, which then fill one L2
+// instead of eight.
+__device__ __forceinline__ int xcd_linear(int L, int T) {
+    const int per = (T + 7) >> 3;
+    const int full = T - (per - 1) * 8;  // 1..8
+    const int x = L & 7, l = L >> 3;
+    return x < full ? x * per + l : full * per + (x - full) * (per - 1) + l;
+}
+
+// Tile (x, y, z) of this block in a GEMM grid, renumbered by xcd_linear with
+// x fastest: the row and band tiles of one K slice run on one XCD.
+struct Tile3 {
+    int x, y, z;
+};
+__device__ __forceinline__ Tile3 gemm_tile_xfirst() {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int j = xcd_linear(L, gx * gy * (int)gridDim.z);
+    const int r = j / gx;
+    return {j - r * gx, r % gy, r / gy};
+}
+// The same for a 2-D grid with y fastest: the band tiles of one block of
+// plane waves (which read the same projector rows) run on one XCD.
+__device__ __forceinline__ Tile3 gemm_tile_yfirst() {
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int j = xcd_linear(blockIdx.x + gx * blockIdx.y, gx * gy);
+    const int x = j / gy;
+    return {x, j - x * gy, 0};
+}
+
 // Complex products use three real MFMA products (the Gauss / 3M form):
 //   conj(a) b :  P1 = ar br, P2 = ai bi, P3 = (ar - ai)(br + bi)
 //                re = P1 + P2,  im = P3 - P1 + P2
@@ -93,11 +128,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
     constexpr int LPB = BN * BK / GEMM_THREADS;   // (psi)
     __shared__ __align__(16) R sA[2][2][BM][KP];  // [buf][re/im][a][k]
     __shared__ __align__(16) R sB[2][2][BN][KP];  // [buf][re/im][n][k]
-    const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const Tile3 tl = gemm_tile_xfirst();
+    const int a0 = tl.x * BM, n0 = tl.y * BN;
     // lower != 0: only the lower triangle (a >= n) of a Hermitian Gram matrix
     // is wanted; tiles wholly above the diagonal are skipped (whole block)
     if (lower && a0 + BM <= n0) return;
-    const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
+    const int kb = tl.z * kslice, ke = min(mtxd, kb + kslice);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int lr = lane & 15, lg = lane >> 4;
@@ -114,25 +150,31 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
     const int arow = threadIdx.x / (BK / LPA), aoff = (threadIdx.x % (BK / LPA)) * LPA;
     const int brow = threadIdx.x / (BK / LPB), boff = (threadIdx.x % (BK / LPB)) * LPB;
     const int ga = a0 + arow, gn = n0 + brow;
-    // branch-free loads: clamped (always valid) address, zero selected outside
+    // rows beyond the matrix (a >= nanl, n >= neig) are read at clamped
+    // addresses and only reach outputs that are not stored; full stages issue
+    // plain loads, the slice tail is clamped with psi zero-masked
     const long abase = (long)min(ga, nanl - 1) * lda, bbase = (long)min(gn, neig - 1) * ldp;
     cplx<R> ra[LPA];
     double2 rb[LPB];
     auto gload = [&](int k0) {
+        if (k0 + BK <= ke) {  // the same for the whole block
 #pragma unroll
-        for (int q = 0; q < LPA; q++) {
-            const int i = k0 + aoff + q;
-            const cplx<R> v = anl[abase + min(i, mtxd - 1)];
-            const bool ok = ga < nanl && i < ke;
-            ra[q] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
-            if (wrow) ra[q] = scale(ra[q], (R)wrow[min(i, mtxd - 1)]);  // per-plane-wave weight
+            for (int q = 0; q < LPA; q++) ra[q] = anl[abase + k0 + aoff + q];
+#pragma unroll
+            for (int q = 0; q < LPB; q++) rb[q] = psi[bbase + k0 + boff + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < LPA; q++) ra[q] = anl[abase + min(k0 + aoff + q, mtxd - 1)];
+#pragma unroll
+            for (int q = 0; q < LPB; q++) {
+                const int i = k0 + boff + q;
+                const double2 v = psi[bbase + min(i, mtxd - 1)];
+                rb[q] = make_double2(i < ke ? v.x : 0.0, i < ke ? v.y : 0.0);
+            }
         }
+        if (wrow) {  // per-plane-wave weight on the projector operand
 #pragma unroll
-        for (int q = 0; q < LPB; q++) {
-            const int i = k0 + boff + q;
-            const double2 v = psi[bbase + min(i, mtxd - 1)];
-            const bool ok = gn < neig && i < ke;
-            rb[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+            for (int q = 0; q < LPA; q++) ra[q] = scale(ra[q], (R)wrow[min(k0 + aoff + q, mtxd - 1)]);
         }
     };
     auto sstore = [&](int buf) {
@@ -191,7 +233,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
         __syncthreads();
         buf ^= 1;
     }
-    cplx<R> *ps = part + (long)blockIdx.z * neig * nanl;
+    cplx<R> *ps = part + (long)tl.z * neig * nanl;
 #pragma unroll
     for (int mi = 0; mi < WM; mi++)
 #pragma unroll
@@ -213,7 +255,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
 // no barrier ever stalls the matrix cores; the waves of a block that share
 // tiles hit the same L1/L2 lines.  Requires anl rows 16-byte aligned (lda
 // even for complex64).  Full chunks use 16-byte vector loads; the slice tail
-// falls back to clamped, zero-masked element loads.
+// falls back to clamped element loads with psi zero-masked.
 // ---------------------------------------------------------------------------
 // Shapes of k_proj_direct (autotuner choices 1..NPROJ_DIRECT): wave tile
 // (WM*16) x (WN*16), KPL K values per lane per chunk, register budget for W
@@ -233,46 +275,50 @@ template <typename R, int V> constexpr int direct_bm() { return 2 * DirectCfg<R,
 template <typename R, int V> constexpr int direct_bn() { return 2 * DirectCfg<R, V>::WN * 16; }
 template <typename R, int V> constexpr int direct_bk() { return 4 * DirectCfg<R, V>::KPL; }
 
+// Chunk loads of the direct kernels.  Rows beyond the matrix (projectors
+// a >= nanl, bands n >= neig) are read at a clamped, valid address and not
+// masked: such a row only reaches output rows or columns that are never
+// stored.  Only the K tail is masked, on one operand (psi), so that the other
+// operand's finite values there multiply zeros.  A full chunk thus issues
+// plain loads whose results are first used by the MFMAs one chunk later,
+// and the load latency stays hidden behind the current chunk.
 template <typename R, int KPL>
-__device__ __forceinline__ void load_row_chunk(cplx<R> (&dst)[KPL], const cplx<R> *__restrict__ row,
-                                               int i0, int ke, int mtxd, bool rowok) {
-    if (rowok && i0 + KPL <= ke) {
-        if constexpr (sizeof(R) == 4) {
+__device__ __forceinline__ void load_row_full(cplx<R> (&dst)[KPL], const cplx<R> *__restrict__ row, int i0) {
+    if constexpr (sizeof(R) == 4) {
 #pragma unroll
-            for (int q = 0; q < KPL / 2; q++) {
-                const float4 f = *reinterpret_cast<const float4 *>(row + i0 + 2 * q);
-                dst[2 * q] = mk<R>(f.x, f.y);
-                dst[2 * q + 1] = mk<R>(f.z, f.w);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < KPL; q++) dst[q] = row[i0 + q];
+        for (int q = 0; q < KPL / 2; q++) {
+            const float4 f = *reinterpret_cast<const float4 *>(row + i0 + 2 * q);
+            dst[2 * q] = mk<R>(f.x, f.y);
+            dst[2 * q + 1] = mk<R>(f.z, f.w);
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < KPL; q++) {
-            const int i = i0 + q;
-            const cplx<R> v = row[min(i, mtxd - 1)];
-            const bool ok = rowok && i < ke;
-            dst[q] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
-        }
+        for (int q = 0; q < KPL; q++) dst[q] = row[i0 + q];
     }
 }
 
+template <typename R, int KPL>
+__device__ __forceinline__ void load_row_tail(cplx<R> (&dst)[KPL], const cplx<R> *__restrict__ row, int i0,
+                                              int mtxd) {
+#pragma unroll
+    for (int q = 0; q < KPL; q++) dst[q] = row[min(i0 + q, mtxd - 1)];
+}
+
 template <int KPL>
-__device__ __forceinline__ void load_psi_chunk(double2 (&dst)[KPL], const double2 *__restrict__ col,
-                                               int i0, int ke, int mtxd, bool colok) {
-    if (colok && i0 + KPL <= ke) {
+__device__ __forceinline__ void load_psi_full(double2 (&dst)[KPL], const double2 *__restrict__ col, int i0) {
 #pragma unroll
-        for (int q = 0; q < KPL; q++) dst[q] = col[i0 + q];
-    } else {
+    for (int q = 0; q < KPL; q++) dst[q] = col[i0 + q];
+}
+
+template <int KPL>
+__device__ __forceinline__ void load_psi_tail(double2 (&dst)[KPL], const double2 *__restrict__ col, int i0,
+                                              int ke, int mtxd) {
 #pragma unroll
-        for (int q = 0; q < KPL; q++) {
-            const int i = i0 + q;
-            const double2 v = col[min(i, mtxd - 1)];
-            const bool ok = colok && i < ke;
-            dst[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
-        }
+    for (int q = 0; q < KPL; q++) {
+        const int i = i0 + q;
+        const double2 v = col[min(i, mtxd - 1)];
+        const bool ok = i < ke;
+        dst[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
     }
 }
 
@@ -285,28 +331,25 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
     constexpr int WM = DirectCfg<R, V>::WM, WN = DirectCfg<R, V>::WN;
     constexpr int KPL = DirectCfg<R, V>::KPL, KC = 4 * KPL;
     constexpr int BM = direct_bm<R, V>(), BN = direct_bn<R, V>();
-    const int a0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const Tile3 tl = gemm_tile_xfirst();
+    const int a0 = tl.x * BM, n0 = tl.y * BN;
     // lower != 0: only the lower triangle (a >= n) of a Hermitian Gram matrix;
     // tiles wholly above the diagonal are skipped (the reduction zeroes them)
     if (lower && a0 + BM <= n0) return;
-    const int kb = blockIdx.z * kslice, ke = min(mtxd, kb + kslice);
+    const int kb = tl.z * kslice, ke = min(mtxd, kb + kslice);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int lr = lane & 15, lg = lane >> 4;
     const cplx<R> *arow[WM];
-    bool aok[WM];
 #pragma unroll
     for (int t = 0; t < WM; t++) {
         const int a = a0 + wm * WM * 16 + t * 16 + lr;
-        aok[t] = a < nanl;
         arow[t] = anl + (long)min(a, nanl - 1) * lda;
     }
     const double2 *bcol[WN];
-    bool bok[WN];
 #pragma unroll
     for (int t = 0; t < WN; t++) {
         const int n = n0 + wn * WN * 16 + t * 16 + lr;
-        bok[t] = n < neig;
         bcol[t] = psi + (long)min(n, neig - 1) * ldp;
     }
     v4 P1[WM][WN], P2[WM][WN], P3[WM][WN];
@@ -320,10 +363,20 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
         }
     cplx<R> ra0[WM][KPL], ra1[WM][KPL];
     double2 rb0[WN][KPL], rb1[WN][KPL];
-    auto load = [&](cplx<R>(&ra)[WM][KPL], double2(&rb)[WN][KPL], int k0) {
+    // tail: the last, partial chunk of the slice
+    auto load = [&](cplx<R>(&ra)[WM][KPL], double2(&rb)[WN][KPL], int k0, bool tail) {
         const int i0 = k0 + lg * KPL;
+        if (!tail) {
 #pragma unroll
-        for (int t = 0; t < WM; t++) load_row_chunk<R, KPL>(ra[t], arow[t], i0, ke, mtxd, aok[t]);
+            for (int t = 0; t < WM; t++) load_row_full<R, KPL>(ra[t], arow[t], i0);
+#pragma unroll
+            for (int t = 0; t < WN; t++) load_psi_full<KPL>(rb[t], bcol[t], i0);
+        } else {
+#pragma unroll
+            for (int t = 0; t < WM; t++) load_row_tail<R, KPL>(ra[t], arow[t], i0, mtxd);
+#pragma unroll
+            for (int t = 0; t < WN; t++) load_psi_tail<KPL>(rb[t], bcol[t], i0, ke, mtxd);
+        }
         if (wrow) {
             // per-plane-wave weight w_i on the projector operand (zeros stay zero)
 #pragma unroll
@@ -333,8 +386,6 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
                 for (int t = 0; t < WM; t++) ra[t][u] = scale(ra[t][u], w);
             }
         }
-#pragma unroll
-        for (int t = 0; t < WN; t++) load_psi_chunk<KPL>(rb[t], bcol[t], i0, ke, mtxd, bok[t]);
     };
     auto compute = [&](const cplx<R>(&ra)[WM][KPL], const double2(&rb)[WN][KPL]) {
 #pragma unroll
@@ -362,15 +413,28 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
                 }
         }
     };
-    if (kb < ke) load(ra0, rb0, kb);
-    for (int k0 = kb; k0 < ke; k0 += 2 * KC) {
-        const bool more1 = k0 + KC < ke, more2 = k0 + 2 * KC < ke;
-        if (more1) load(ra1, rb1, k0 + KC);
-        compute(ra0, rb0);
-        if (more2) load(ra0, rb0, k0 + 2 * KC);
-        if (more1) compute(ra1, rb1);
+    // Full chunks, two per trip: the loads of the next chunk are issued
+    // unconditionally (at the last full chunk again when there is none, a
+    // valid address whose values go unused) so that the loop body is one
+    // straight block and each chunk's loads are waited for only after the
+    // other chunk's MFMAs.  Then the partial chunk, if any.
+    const int nfull = max(0, ke - kb) / KC;
+    if (nfull > 0) {
+        load(ra0, rb0, kb, false);
+        int c = 0;
+        for (; c + 2 <= nfull; c += 2) {
+            load(ra1, rb1, kb + (c + 1) * KC, false);
+            compute(ra0, rb0);
+            load(ra0, rb0, kb + min(c + 2, nfull - 1) * KC, false);
+            compute(ra1, rb1);
+        }
+        if (c < nfull) compute(ra0, rb0);
     }
-    cplx<R> *ps = part + (long)blockIdx.z * neig * nanl;
+    if (kb + nfull * KC < ke) {
+        load(ra0, rb0, kb + nfull * KC, true);
+        compute(ra0, rb0);
+    }
+    cplx<R> *ps = part + (long)tl.z * neig * nanl;
 #pragma unroll
     for (int mi = 0; mi < WM; mi++)
 #pragma unroll
@@ -474,8 +538,9 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
     __shared__ __align__(16) R sD[3][BP_BN][KCP];   // re, im, re+im
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    const int i = blockIdx.x * BP_BI + wave * 16 + lr;
-    const int n0 = blockIdx.y * BP_BN;
+    const Tile3 tl = gemm_tile_yfirst();
+    const int i = tl.x * BP_BI + wave * 16 + lr;
+    const int n0 = tl.y * BP_BN;
     v4 P1[4], P2[4], P3[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -486,27 +551,33 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
     // staging of dhd: thread -> band row tid/4, projector offset (tid%4)*DPT
     const int srow = threadIdx.x >> 2, soff = (threadIdx.x & 3) * DPT;
     cplx<R> rd[DPT];
-    // branch-free loads: clamped (always valid) address, zero selected outside
-    const int nrow = n0 + srow;
-    const long dbase = (long)min(nrow, neig - 1) * nanl;
+    // rows beyond the matrix (bands n >= neig, plane waves i >= mtxd) are
+    // read at clamped addresses and only reach outputs that are not stored;
+    // full chunks issue plain loads (first used after the current chunk's
+    // MFMAs), the projector tail is clamped with dhd zero-masked
+    const long dbase = (long)min(n0 + srow, neig - 1) * nanl;
     auto dload = [&](int ka) {
+        if (ka + KC <= nanl) {  // the same for the whole block
 #pragma unroll
-        for (int q = 0; q < DPT; q++) {
-            const int a = ka + soff + q;
-            const cplx<R> v = dhd[dbase + min(a, nanl - 1)];
-            const bool ok = nrow < neig && a < nanl;
-            rd[q] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+            for (int q = 0; q < DPT; q++) rd[q] = dhd[dbase + ka + soff + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < DPT; q++) {
+                const int a = ka + soff + q;
+                const cplx<R> v = dhd[dbase + min(a, nanl - 1)];
+                rd[q] = mk<R>(a < nanl ? v.x : (R)0, a < nanl ? v.y : (R)0);
+            }
         }
     };
     cplx<R> av[KPL];
-    const int ic = min(i, mtxd - 1);
+    const cplx<R> *acol = anl + min(i, mtxd - 1);
     auto aload = [&](int ka) {
+        if (ka + KC <= nanl) {
 #pragma unroll
-        for (int u = 0; u < KPL; u++) {
-            const int a = ka + lg * KPL + u;
-            const cplx<R> v = anl[(long)min(a, nanl - 1) * lda + ic];
-            const bool ok = i < mtxd && a < nanl;
-            av[u] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+            for (int u = 0; u < KPL; u++) av[u] = acol[(long)(ka + lg * KPL + u) * lda];
+        } else {
+#pragma unroll
+            for (int u = 0; u < KPL; u++) av[u] = acol[(long)min(ka + lg * KPL + u, nanl - 1) * lda];
         }
     };
     dload(0);
@@ -606,25 +677,23 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
     constexpr int KPL = BackDirectCfg<R, V>::KPL, KC = 4 * KPL;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
-    const int i0 = blockIdx.x * back_bi<R, V>() + wave * 16 * IT;
-    const int n0 = blockIdx.y * back_bn<R, V>();
-    int ic[IT];
+    const Tile3 tl = gemm_tile_yfirst();
+    const int i0 = tl.x * back_bi<R, V>() + wave * 16 * IT;
+    const int n0 = tl.y * back_bn<R, V>();
+    // projector values of this lane's plane waves (column i of anlga^T) and
+    // dhd rows (bands) of its NT row tiles, at clamped addresses: a clamped
+    // row only reaches outputs that are not stored (see load_row_full)
     bool iok[IT];
+    const cplx<R> *acol[IT];
 #pragma unroll
     for (int q = 0; q < IT; q++) {
         const int i = i0 + q * 16 + lr;
         iok[q] = i < mtxd;
-        ic[q] = min(i, mtxd - 1);
+        acol[q] = anl + min(i, mtxd - 1);
     }
-    // dhd rows (bands) of this lane for the NT row tiles, clamped
     const cplx<R> *drow[NT];
-    bool dok[NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int n = n0 + t * 16 + lr;
-        dok[t] = n < neig;
-        drow[t] = dhd + (long)min(n, neig - 1) * nanl;
-    }
+    for (int t = 0; t < NT; t++) drow[t] = dhd + (long)min(n0 + t * 16 + lr, neig - 1) * nanl;
     v4 P1[NT][IT], P2[NT][IT], P3[NT][IT];
 #pragma unroll
     for (int t = 0; t < NT; t++)
@@ -635,23 +704,30 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
             P3[t][q] = v4{0, 0, 0, 0};
         }
     cplx<R> d0[NT][KPL], d1[NT][KPL], a0[IT][KPL], a1[IT][KPL];
-    auto load = [&](cplx<R>(&d)[NT][KPL], cplx<R>(&av)[IT][KPL], int ka) {
+    // full chunks: plain loads; the projector tail: clamped, dhd zero-masked
+    auto load = [&](cplx<R>(&d)[NT][KPL], cplx<R>(&av)[IT][KPL], int ka, bool tail) {
+        if (!tail) {
 #pragma unroll
-        for (int u = 0; u < KPL; u++) {
-            const int a = ka + lg * KPL + u;
-            const int acl = min(a, nanl - 1);
-            const bool aok = a < nanl;
+            for (int u = 0; u < KPL; u++) {
+                const int a = ka + lg * KPL + u;
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const cplx<R> v = drow[t][acl];
-                const bool ok = aok && dok[t];
-                d[t][u] = mk<R>(ok ? v.x : (R)0, ok ? v.y : (R)0);
+                for (int t = 0; t < NT; t++) d[t][u] = drow[t][a];
+#pragma unroll
+                for (int q = 0; q < IT; q++) av[q][u] = acol[q][(long)a * lda];
             }
+        } else {
 #pragma unroll
-            for (int q = 0; q < IT; q++) {
-                const cplx<R> w = anl[(long)acl * lda + ic[q]];
-                const bool ok = aok && iok[q];
-                av[q][u] = mk<R>(ok ? w.x : (R)0, ok ? w.y : (R)0);
+            for (int u = 0; u < KPL; u++) {
+                const int a = ka + lg * KPL + u;
+                const int acl = min(a, nanl - 1);
+                const bool aok = a < nanl;
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const cplx<R> v = drow[t][acl];
+                    d[t][u] = mk<R>(aok ? v.x : (R)0, aok ? v.y : (R)0);
+                }
+#pragma unroll
+                for (int q = 0; q < IT; q++) av[q][u] = acol[q][(long)acl * lda];
             }
         }
     };
@@ -677,13 +753,23 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
             }
         }
     };
-    load(d0, a0, 0);
-    for (int ka = 0; ka < nanl; ka += 2 * KC) {
-        const bool more1 = ka + KC < nanl, more2 = ka + 2 * KC < nanl;
-        if (more1) load(d1, a1, ka + KC);
+    // full chunks two per trip with unconditional (clamped) prefetch, as in
+    // k_proj_direct, then the partial chunk
+    const int nfull = nanl / KC;
+    if (nfull > 0) {
+        load(d0, a0, 0, false);
+        int c = 0;
+        for (; c + 2 <= nfull; c += 2) {
+            load(d1, a1, (c + 1) * KC, false);
+            compute(d0, a0);
+            load(d0, a0, min(c + 2, nfull - 1) * KC, false);
+            compute(d1, a1);
+        }
+        if (c < nfull) compute(d0, a0);
+    }
+    if (nfull * KC < nanl) {
+        load(d0, a0, nfull * KC, true);
         compute(d0, a0);
-        if (more2) load(d0, a0, ka + 2 * KC);
-        if (more1) compute(d1, a1);
     }
 #pragma unroll
     for (int q = 0; q < IT; q++) {

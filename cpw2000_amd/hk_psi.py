@@ -61,6 +61,11 @@ TUNE_PLANE, TUNE_SPLITK, TUNE_PROJ, TUNE_BACK, TUNE_KIN, TUNE_SPIN, TUNE_COL, TU
 # with wave-exchanged LDS y lines, 8-11 register shapes that skip the slots
 # outside the sphere box (meshes with the box inside N/4)
 NPLANE_VAR = 12
+# projection kernels (TUNE_PROJ): 0 LDS-staged, 1-4 direct-load shapes, 5
+# LDS-staged with half-size wave tiles; back-projection kernels (TUNE_BACK):
+# 0 LDS-staged, 1-3 direct-load shapes, 4 LDS-staged with half-size chunks
+NPROJ_VAR = 6
+NBACK_VAR = 5
 
 # kernel ids of the measurement hook (CPWGPU_K_* in include/cpwgpu.h)
 KERNELS = ("k_col_bwd", "k_plane", "k_col_fwd", "k_proj", "k_proj_reduce", "k_back", "k_proj_kb")
@@ -423,14 +428,17 @@ class Plan:
 
     def set_variant(self, prec, what, value):
         """Kernel choice for a precision: what = TUNE_PLANE (plane-stage kernel,
-        0 LDS two-pass, 1..3 register shapes, -1 default), TUNE_PROJ
+        0 .. NPLANE_VAR-1 as listed at NPLANE_VAR, -1 default), TUNE_PROJ
         (projection GEMM, 0 LDS-staged, 1..4 direct-load shapes, 5 LDS-staged
         with half-size wave tiles), TUNE_BACK (back-projection GEMM, 0
         LDS-staged, 1..3 direct-load shapes, 4 LDS-staged with half-size chunks),
         TUNE_SPLITK (projection GEMM split-K target blocks, -1 default),
         TUNE_KIN (kinetic term in the back-projection epilogue 0 or in the
-        column gather 1) or TUNE_SPIN (plane kernel of apply_spin with nsp = 4:
-        0 LDS two-pass, 1 register lines where built, -1 default)."""
+        column gather 1), TUNE_SPIN (plane kernel of apply_spin with nsp = 4:
+        0 LDS two-pass, 1 register lines where built, -1 default), TUNE_COL
+        (z-column kernels: 0 LDS two-pass, 1 register lines where built) or
+        TUNE_SCHED (stream schedule: 0 the whole non-local term beside the
+        local one, 1 projection beside it and back-projection after it)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 
     def get_variant(self, prec, what):
