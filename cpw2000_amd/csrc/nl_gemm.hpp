@@ -260,17 +260,20 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_proj(const cplx<R> *__restrict
 // Shapes of k_proj_direct (autotuner choices 1..NPROJ_DIRECT): wave tile
 // (WM*16) x (WN*16), KPL K values per lane per chunk, register budget for W
 // waves per SIMD (accumulators plus two chunks of loads).
+// register buffers in the chunk ring of the direct kernels (2: double
+// buffering; a third buffer measured no faster on gfx950)
+constexpr int DIRECT_D = 2;
 constexpr int NPROJ_DIRECT = 4;
 template <typename R, int V> struct DirectCfg;
 // (all compile without register spills on gfx950)
-template <> struct DirectCfg<float, 1> { static constexpr int WM = 4, WN = 2, KPL = 2, W = 2; };
-template <> struct DirectCfg<float, 2> { static constexpr int WM = 4, WN = 2, KPL = 4, W = 1; };
-template <> struct DirectCfg<float, 3> { static constexpr int WM = 2, WN = 2, KPL = 8, W = 1; };
-template <> struct DirectCfg<float, 4> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 2; };
-template <> struct DirectCfg<double, 1> { static constexpr int WM = 2, WN = 2, KPL = 2, W = 2; };
-template <> struct DirectCfg<double, 2> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 1; };
-template <> struct DirectCfg<double, 3> { static constexpr int WM = 4, WN = 1, KPL = 2, W = 2; };
-template <> struct DirectCfg<double, 4> { static constexpr int WM = 2, WN = 1, KPL = 4, W = 2; };
+template <> struct DirectCfg<float, 1> { static constexpr int WM = 4, WN = 2, KPL = 2, W = 2, D = DIRECT_D; };
+template <> struct DirectCfg<float, 2> { static constexpr int WM = 4, WN = 2, KPL = 4, W = 1, D = DIRECT_D; };
+template <> struct DirectCfg<float, 3> { static constexpr int WM = 2, WN = 2, KPL = 8, W = 1, D = DIRECT_D; };
+template <> struct DirectCfg<float, 4> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 2, D = DIRECT_D; };
+template <> struct DirectCfg<double, 1> { static constexpr int WM = 2, WN = 2, KPL = 2, W = 2, D = DIRECT_D; };
+template <> struct DirectCfg<double, 2> { static constexpr int WM = 2, WN = 2, KPL = 4, W = 1, D = DIRECT_D; };
+template <> struct DirectCfg<double, 3> { static constexpr int WM = 4, WN = 1, KPL = 2, W = 2, D = DIRECT_D; };
+template <> struct DirectCfg<double, 4> { static constexpr int WM = 2, WN = 1, KPL = 4, W = 2, D = DIRECT_D; };
 template <typename R, int V> constexpr int direct_bm() { return 2 * DirectCfg<R, V>::WM * 16; }
 template <typename R, int V> constexpr int direct_bn() { return 2 * DirectCfg<R, V>::WN * 16; }
 template <typename R, int V> constexpr int direct_bk() { return 4 * DirectCfg<R, V>::KPL; }
@@ -361,21 +364,22 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
             P2[i][j] = v4{0, 0, 0, 0};
             P3[i][j] = v4{0, 0, 0, 0};
         }
-    cplx<R> ra0[WM][KPL], ra1[WM][KPL];
-    double2 rb0[WN][KPL], rb1[WN][KPL];
+    constexpr int D = DirectCfg<R, V>::D;
+    cplx<R> ra[D][WM][KPL];
+    double2 rb[D][WN][KPL];
     // tail: the last, partial chunk of the slice
-    auto load = [&](cplx<R>(&ra)[WM][KPL], double2(&rb)[WN][KPL], int k0, bool tail) {
+    auto load = [&](cplx<R>(&ra_)[WM][KPL], double2(&rb_)[WN][KPL], int k0, bool tail) {
         const int i0 = k0 + lg * KPL;
         if (!tail) {
 #pragma unroll
-            for (int t = 0; t < WM; t++) load_row_full<R, KPL>(ra[t], arow[t], i0);
+            for (int t = 0; t < WM; t++) load_row_full<R, KPL>(ra_[t], arow[t], i0);
 #pragma unroll
-            for (int t = 0; t < WN; t++) load_psi_full<KPL>(rb[t], bcol[t], i0);
+            for (int t = 0; t < WN; t++) load_psi_full<KPL>(rb_[t], bcol[t], i0);
         } else {
 #pragma unroll
-            for (int t = 0; t < WM; t++) load_row_tail<R, KPL>(ra[t], arow[t], i0, mtxd);
+            for (int t = 0; t < WM; t++) load_row_tail<R, KPL>(ra_[t], arow[t], i0, mtxd);
 #pragma unroll
-            for (int t = 0; t < WN; t++) load_psi_tail<KPL>(rb[t], bcol[t], i0, ke, mtxd);
+            for (int t = 0; t < WN; t++) load_psi_tail<KPL>(rb_[t], bcol[t], i0, ke, mtxd);
         }
         if (wrow) {
             // per-plane-wave weight w_i on the projector operand (zeros stay zero)
@@ -383,24 +387,24 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
             for (int u = 0; u < KPL; u++) {
                 const R w = (R)wrow[min(i0 + u, mtxd - 1)];
 #pragma unroll
-                for (int t = 0; t < WM; t++) ra[t][u] = scale(ra[t][u], w);
+                for (int t = 0; t < WM; t++) ra_[t][u] = scale(ra_[t][u], w);
             }
         }
     };
-    auto compute = [&](const cplx<R>(&ra)[WM][KPL], const double2(&rb)[WN][KPL]) {
+    auto compute = [&](const cplx<R>(&ra_)[WM][KPL], const double2(&rb_)[WN][KPL]) {
 #pragma unroll
         for (int u = 0; u < KPL; u++) {
             R ar[WM], ai[WM], as[WM], br[WN], bi[WN], bs[WN];
 #pragma unroll
             for (int t = 0; t < WM; t++) {
-                ar[t] = ra[t][u].x;
-                ai[t] = ra[t][u].y;
+                ar[t] = ra_[t][u].x;
+                ai[t] = ra_[t][u].y;
                 as[t] = ar[t] - ai[t];
             }
 #pragma unroll
             for (int t = 0; t < WN; t++) {
-                br[t] = (R)rb[t][u].x;
-                bi[t] = (R)rb[t][u].y;
+                br[t] = (R)rb_[t][u].x;
+                bi[t] = (R)rb_[t][u].y;
                 bs[t] = br[t] + bi[t];
             }
 #pragma unroll
@@ -413,26 +417,31 @@ __global__ __launch_bounds__(GEMM_THREADS, (DirectCfg<R, V>::W)) void k_proj_dir
                 }
         }
     };
-    // Full chunks, two per trip: the loads of the next chunk are issued
-    // unconditionally (at the last full chunk again when there is none, a
-    // valid address whose values go unused) so that the loop body is one
-    // straight block and each chunk's loads are waited for only after the
-    // other chunk's MFMAs.  Then the partial chunk, if any.
+    // Full chunks through a ring of D register buffers, D per trip: chunk
+    // c + D - 1 is loaded while chunk c is multiplied, unconditionally (at
+    // the last full chunk again when there is none: a valid address whose
+    // values go unused), so that the loop body is one straight block and each
+    // chunk's loads have D - 1 chunks of MFMAs to arrive.  Then the partial
+    // chunk, if any.  Chunks are accumulated in K order for any D.
     const int nfull = max(0, ke - kb) / KC;
     if (nfull > 0) {
-        load(ra0, rb0, kb, false);
+#pragma unroll
+        for (int j = 0; j < D - 1; j++) load(ra[j], rb[j], kb + min(j, nfull - 1) * KC, false);
         int c = 0;
-        for (; c + 2 <= nfull; c += 2) {
-            load(ra1, rb1, kb + (c + 1) * KC, false);
-            compute(ra0, rb0);
-            load(ra0, rb0, kb + min(c + 2, nfull - 1) * KC, false);
-            compute(ra1, rb1);
+        for (; c + D <= nfull; c += D) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                load(ra[(j + D - 1) % D], rb[(j + D - 1) % D], kb + min(c + j + D - 1, nfull - 1) * KC, false);
+                compute(ra[j], rb[j]);
+            }
         }
-        if (c < nfull) compute(ra0, rb0);
+#pragma unroll
+        for (int j = 0; j < D - 1; j++)
+            if (c + j < nfull) compute(ra[j], rb[j]);
     }
     if (kb + nfull * KC < ke) {
-        load(ra0, rb0, kb + nfull * KC, true);
-        compute(ra0, rb0);
+        load(ra[0], rb[0], kb + nfull * KC, true);
+        compute(ra[0], rb[0]);
     }
     cplx<R> *ps = part + (long)tl.z * neig * nanl;
 #pragma unroll
@@ -658,12 +667,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
 constexpr int NBACK_DIRECT = 3;
 template <typename R, int V> struct BackDirectCfg;
 // (all compile without register spills on gfx950)
-template <> struct BackDirectCfg<float, 1> { static constexpr int NT = 4, IT = 1, KPL = 4, W = 2; };
-template <> struct BackDirectCfg<float, 2> { static constexpr int NT = 4, IT = 2, KPL = 4, W = 2; };
-template <> struct BackDirectCfg<float, 3> { static constexpr int NT = 2, IT = 2, KPL = 4, W = 3; };
-template <> struct BackDirectCfg<double, 1> { static constexpr int NT = 4, IT = 1, KPL = 2, W = 2; };
-template <> struct BackDirectCfg<double, 2> { static constexpr int NT = 2, IT = 2, KPL = 2, W = 2; };
-template <> struct BackDirectCfg<double, 3> { static constexpr int NT = 2, IT = 1, KPL = 2, W = 3; };
+template <> struct BackDirectCfg<float, 1> { static constexpr int NT = 4, IT = 1, KPL = 4, W = 2, D = DIRECT_D; };
+template <> struct BackDirectCfg<float, 2> { static constexpr int NT = 4, IT = 2, KPL = 4, W = 2, D = DIRECT_D; };
+template <> struct BackDirectCfg<float, 3> { static constexpr int NT = 2, IT = 2, KPL = 4, W = 3, D = DIRECT_D; };
+template <> struct BackDirectCfg<double, 1> { static constexpr int NT = 4, IT = 1, KPL = 2, W = 2, D = DIRECT_D; };
+template <> struct BackDirectCfg<double, 2> { static constexpr int NT = 2, IT = 2, KPL = 2, W = 2, D = DIRECT_D; };
+template <> struct BackDirectCfg<double, 3> { static constexpr int NT = 2, IT = 1, KPL = 2, W = 3, D = DIRECT_D; };
 template <typename R, int V> constexpr int back_bi() { return 4 * 16 * BackDirectCfg<R, V>::IT; }
 template <typename R, int V> constexpr int back_bn() { return 16 * BackDirectCfg<R, V>::NT; }
 
@@ -703,9 +712,10 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
             P2[t][q] = v4{0, 0, 0, 0};
             P3[t][q] = v4{0, 0, 0, 0};
         }
-    cplx<R> d0[NT][KPL], d1[NT][KPL], a0[IT][KPL], a1[IT][KPL];
+    constexpr int D = BackDirectCfg<R, V>::D;
+    cplx<R> dv[D][NT][KPL], av[D][IT][KPL];
     // full chunks: plain loads; the projector tail: clamped, dhd zero-masked
-    auto load = [&](cplx<R>(&d)[NT][KPL], cplx<R>(&av)[IT][KPL], int ka, bool tail) {
+    auto load = [&](cplx<R>(&d)[NT][KPL], cplx<R>(&a_)[IT][KPL], int ka, bool tail) {
         if (!tail) {
 #pragma unroll
             for (int u = 0; u < KPL; u++) {
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
 #pragma unroll
                 for (int t = 0; t < NT; t++) d[t][u] = drow[t][a];
 #pragma unroll
-                for (int q = 0; q < IT; q++) av[q][u] = acol[q][(long)a * lda];
+                for (int q = 0; q < IT; q++) a_[q][u] = acol[q][(long)a * lda];
             }
         } else {
 #pragma unroll
@@ -727,19 +737,19 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
                     d[t][u] = mk<R>(aok ? v.x : (R)0, aok ? v.y : (R)0);
                 }
 #pragma unroll
-                for (int q = 0; q < IT; q++) av[q][u] = acol[q][(long)acl * lda];
+                for (int q = 0; q < IT; q++) a_[q][u] = acol[q][(long)acl * lda];
             }
         }
     };
-    auto compute = [&](const cplx<R>(&d)[NT][KPL], const cplx<R>(&av)[IT][KPL]) {
+    auto compute = [&](const cplx<R>(&d)[NT][KPL], const cplx<R>(&a_)[IT][KPL]) {
 #pragma unroll
         for (int u = 0; u < KPL; u++) {
             R cr[IT], ci[IT], cs[IT];
 #pragma unroll
             for (int q = 0; q < IT; q++) {
-                cr[q] = av[q][u].x;
-                ci[q] = av[q][u].y;
-                cs[q] = av[q][u].x + av[q][u].y;
+                cr[q] = a_[q][u].x;
+                ci[q] = a_[q][u].y;
+                cs[q] = a_[q][u].x + a_[q][u].y;
             }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
@@ -753,23 +763,27 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
             }
         }
     };
-    // full chunks two per trip with unconditional (clamped) prefetch, as in
-    // k_proj_direct, then the partial chunk
+    // full chunks through a ring of D register buffers with unconditional
+    // (clamped) prefetch, as in k_proj_direct, then the partial chunk
     const int nfull = nanl / KC;
     if (nfull > 0) {
-        load(d0, a0, 0, false);
+#pragma unroll
+        for (int j = 0; j < D - 1; j++) load(dv[j], av[j], min(j, nfull - 1) * KC, false);
         int c = 0;
-        for (; c + 2 <= nfull; c += 2) {
-            load(d1, a1, (c + 1) * KC, false);
-            compute(d0, a0);
-            load(d0, a0, min(c + 2, nfull - 1) * KC, false);
-            compute(d1, a1);
+        for (; c + D <= nfull; c += D) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                load(dv[(j + D - 1) % D], av[(j + D - 1) % D], min(c + j + D - 1, nfull - 1) * KC, false);
+                compute(dv[j], av[j]);
+            }
         }
-        if (c < nfull) compute(d0, a0);
+#pragma unroll
+        for (int j = 0; j < D - 1; j++)
+            if (c + j < nfull) compute(dv[j], av[j]);
     }
     if (nfull * KC < nanl) {
-        load(d0, a0, nfull * KC, true);
-        compute(d0, a0);
+        load(dv[0], av[0], nfull * KC, true);
+        compute(dv[0], av[0]);
     }
 #pragma unroll
     for (int q = 0; q < IT; q++) {
