@@ -498,7 +498,11 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
         const FftGeom &g = p->g;
         if (PR && !prune_fits(g))
             return fail(CPWGPU_ERR_MODE, "plane variant %d needs the sphere box inside N/4", V);
-        const int S = reg_stride(g.nx);
+        // pruned shapes use their compile-time row stride (the box has at
+        // most N/2 + 1 columns there)
+        const int S = PR ? pr_sub_stride<N>() : reg_stride(g.nx);
+        if (reg_stride(g.nx) > S)
+            return fail(CPWGPU_ERR_MODE, "plane variant %d: %d box columns exceed its row stride", V, g.nx);
         const size_t dyn = plane_reg_lds<R, N, P>(S);
         if (dyn > 160 * 1024)
             return fail(CPWGPU_ERR_SIZE, "plane variant %d: %zu bytes of LDS", V, dyn);

@@ -154,16 +154,22 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 // PR: the sphere box is |x|, |y| <= N/4 (checked by the caller), so the
 // register slots that hold only x (y) outside the box on every lane are known
 // zeros of the inverse transforms and unused outputs of the forward ones:
-// they are neither loaded, transformed nor stored.
+// they are neither loaded, transformed nor stored.  The box then has at most
+// N/2 + 1 columns, and the sub-plane row stride is the compile-time
+// pr_sub_stride<N>() instead of the argument S: every LDS offset of the y
+// lines and x rows folds into the instructions' immediate offsets.
 // ---------------------------------------------------------------------------
+template <int N> constexpr int pr_sub_stride() { return N / 2 + 3; }
+
 template <typename R, int N, int GX, int GY, int P, int T, int W, bool YW = false, bool PR = false>
 __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf,
                                                  const int *__restrict__ col_uy,
                                                  const R *__restrict__ vr,
                                                  const double2 *__restrict__ twg, FftGeom g,
-                                                 int nb, int S) {
+                                                 int nb, int S_arg) {
     static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
     constexpr int MX = N / GX, MY = N / GY;
+    const int S = PR ? pr_sub_stride<N>() : S_arg;
     // slots that can hold box points (all of them unless PR)
     constexpr u64mask XB = PR ? box_slots(N, GX, MX, N / 4) : full_mask(MX);
     constexpr u64mask YB = PR ? box_slots(N, GY, MY, N / 4) : full_mask(MY);
