@@ -1296,19 +1296,27 @@ int launch_plane_spin_reg(cpwgpu_plan_s *p, cplx<R> *colbuf, const R *vr, long v
         return fail(CPWGPU_ERR_MODE, "register two-component plane kernel is not built for N = %d", N);
     } else {
         const FftGeom &g = p->g;
-        const int S = reg_stride(g.nx);
+        // a box inside N/4: the compile-time row stride
+        const bool cs = prune_fits(g);
+        const int S = cs ? pr_sub_stride<N>() : reg_stride(g.nx);
         const size_t dyn = sizeof(cplx<R>) * ((size_t)N + 2 * (size_t)N * S);
         if (dyn > 160 * 1024)
             return fail(CPWGPU_ERR_SIZE, "two-component plane of %d x %d does not fit in LDS", N, N);
         static bool attr = false;
         if (!attr) {
-            HIPCHK(hipFuncSetAttribute((const void *)k_plane_spin_reg<R, N, GX, GY, T, W>,
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_spin_reg<R, N, GX, GY, T, W, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            HIPCHK(hipFuncSetAttribute((const void *)k_plane_spin_reg<R, N, GX, GY, T, W, true>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
         ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane_spin_reg<R, N, GX, GY, T, W>), dim3(g.n[2] * npair), dim3(T), dyn, s, colbuf,
-                           (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p, g, npair, S);
+        if (cs)
+            hipLaunchKernelGGL((k_plane_spin_reg<R, N, GX, GY, T, W, true>), dim3(g.n[2] * npair), dim3(T), dyn, s,
+                               colbuf, (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p, g, npair, S);
+        else
+            hipLaunchKernelGGL((k_plane_spin_reg<R, N, GX, GY, T, W, false>), dim3(g.n[2] * npair), dim3(T), dyn, s,
+                               colbuf, (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p, g, npair, S);
         HIPCHK(hipGetLastError());
         return CPWGPU_OK;
     }
@@ -2222,18 +2230,27 @@ int launch_density(cpwgpu_plan_s *p, const cplx<double> *colbuf, const double *o
         if (rhor) {
             // the density planes alone: register lines, accumulation in registers
             constexpr int DT = den_threads<N>();
-            const int S = reg_stride(g.nx);
+            // a box inside N/4: the compile-time row stride
+            const bool cs = prune_fits(g);
+            const int S = cs ? pr_sub_stride<N>() : reg_stride(g.nx);
             const size_t dyn = 16 * ((size_t)N + (size_t)N * S);
             if (dyn > 160 * 1024) return fail(CPWGPU_ERR_SIZE, "density plane %d does not fit in LDS", N);
             static bool attr_r = false;
             if (!attr_r) {
-                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W>,
+                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                HIPCHK(hipFuncSetAttribute((const void *)k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W, true>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                 attr_r = true;
             }
-            hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W>), dim3(g.n[2] * ngrp), dim3(DT),
-                               dyn, s, colbuf, (const int *)p->col_uy.p, occ, (const double2 *)p->tw[0].p, g, nb,
-                               bpg, S, rhor);
+            if (cs)
+                hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W, true>), dim3(g.n[2] * ngrp),
+                                   dim3(DT), dyn, s, colbuf, (const int *)p->col_uy.p, occ,
+                                   (const double2 *)p->tw[0].p, g, nb, bpg, S, rhor);
+            else
+                hipLaunchKernelGGL((k_density_reg<N, DEN_GX, DEN_GY, DT, DEN_W, false>), dim3(g.n[2] * ngrp),
+                                   dim3(DT), dyn, s, colbuf, (const int *)p->col_uy.p, occ,
+                                   (const double2 *)p->tw[0].p, g, nb, bpg, S, rhor);
             HIPCHK(hipGetLastError());
             return CPWGPU_OK;
         }

@@ -146,14 +146,17 @@ __global__ __launch_bounds__(FT) void k_density_fast(const cplx<double> *__restr
 // transform), and each block writes its rho(x, y) plane once:
 // rhor[grp][z][y][x], summed over groups in a fixed order afterwards.
 // Grid: n3 * ceil(nb / bpg) blocks of T threads.
-template <int N, int GX, int GY, int T, int W>
+// CS: the sphere box lies inside N/4 (checked by the caller) and the row
+// stride is the compile-time pr_sub_stride<N>() instead of the argument.
+template <int N, int GX, int GY, int T, int W, bool CS = false>
 __global__ __launch_bounds__(T, W) void k_density_reg(const cplx<double> *__restrict__ colbuf,
                                                    const int *__restrict__ col_uy,
                                                    const double *__restrict__ occ,
                                                    const double2 *__restrict__ twg, FftGeom g, int nb,
-                                                   int bpg, int S, double *__restrict__ rhor) {
+                                                   int bpg, int S_arg, double *__restrict__ rhor) {
     typedef double R;
     static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
+    const int S = CS ? pr_sub_stride<N>() : S_arg;
     constexpr int MX = N / GX, MY = N / GY;
     constexpr int TX = T / GX;                 // x-row tasks per round
     constexpr int RPT = (N + TX - 1) / TX;     // rows per x task

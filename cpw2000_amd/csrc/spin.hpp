@@ -208,14 +208,17 @@ __global__ __launch_bounds__(FT) void k_plane_spin(cplx<R> *__restrict__ colbuf,
 // M consecutive x points of the two components and applies the 2x2 potential
 // there (vsp = [V1 + V2, V1 - V2, V3, V4], stride vstride), then both forward
 // transforms.  Grid: n3 * npair blocks of T threads, one (z, band pair) each.
-template <typename R, int N, int GX, int GY, int T, int W>
+// CS: the sphere box lies inside N/4 (checked by the caller) and the row
+// stride is the compile-time pr_sub_stride<N>() instead of the argument.
+template <typename R, int N, int GX, int GY, int T, int W, bool CS = false>
 __global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ colbuf,
                                                       const int *__restrict__ col_uy,
                                                       const R *__restrict__ vsp, long vstride,
                                                       const double2 *__restrict__ twg, FftGeom g,
-                                                      int npair, int S) {
+                                                      int npair, int S_arg) {
     static_assert(N % GX == 0 && N % GY == 0 && T % 64 == 0, "bad plane shape");
     constexpr int MX = N / GX, MY = N / GY;
+    const int S = CS ? pr_sub_stride<N>() : S_arg;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<R> *twN = reinterpret_cast<cplx<R> *>(smem_raw);  // [N]
     cplx<R> *sub = twN + N;                                 // [2][N][S]: up, down
