@@ -2526,19 +2526,29 @@ int time_apply(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
         return spin ? cpwgpu_plan_apply_spin(p, neig, psi, ldp, hpsi, ldh, prec, s)
                     : cpwgpu_plan_apply(p, neig, psi, ldp, hpsi, ldh, prec, s);
     };
+    // batches of back-to-back applies, as a caller's loop runs them (the
+    // next apply's first kernels overlap the previous one's last): the time
+    // per apply of the fastest batch; applies of more than 2 ms are timed
+    // one by one (their ends are a small part of them)
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, s));
     int st = apply();
+    HIPCHK(hipEventRecord(b, s));
+    HIPCHK(hipEventSynchronize(b));
+    float first = 0.f;
+    HIPCHK(hipEventElapsedTime(&first, a, b));
+    const int BATCH = first > 2.f ? 1 : 3;
     *best = 3.4e38f;
     for (int r = 0; r < reps && !st; r++) {
         HIPCHK(hipEventRecord(a, s));
-        st = apply();
+        for (int i = 0; i < BATCH && !st; i++) st = apply();
         HIPCHK(hipEventRecord(b, s));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, a, b));
-        *best = std::min(*best, ms);
+        *best = std::min(*best, ms / BATCH);
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
