@@ -95,17 +95,32 @@ template <int DIR, typename R> __device__ __forceinline__ cplx<R> cmul_dir(cplx<
     return cmul(a, DIR > 0 ? w : mk<R>(w.x, -w.y));
 }
 
-// DIF transform of a G-lane line (layouts above); twN[e] = exp(+2 pi i e/N).
-// IN: the register slots j that can hold a nonzero value (the others are zero
-// on every lane and are not read).
-template <int DIR, int G, int M, typename R, u64mask IN = full_mask(M)>
-__device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
-                                         const cplx<R> *twN) {
+// DIF transform of a G-lane line (layouts above) with the lane twiddles
+// tw(k) = exp(+2 pi i g k / N), k = 1..M-1.  IN: the register slots j that can
+// hold a nonzero value (the others are zero on every lane and are not read).
+// Lane twiddle sources: TwLds reads twN[g k] from LDS at each use (its index
+// is made opaque where begin() is called, so the loads stay at their uses and
+// nothing is held in registers across transforms); TwRegs returns values
+// already in registers.
+template <typename R> struct TwLds {
+    const cplx<R> *twN;
+    int g, gi = 0;
+    __device__ __forceinline__ void begin() { gi = opaque(g); }
+    __device__ __forceinline__ cplx<R> operator()(int k) const { return twN[gi * k]; }
+};
+template <typename R, int M> struct TwRegs {
+    const cplx<R> (&tw)[M];
+    __device__ __forceinline__ void begin() {}
+    __device__ __forceinline__ cplx<R> operator()(int k) const { return tw[k]; }
+};
+
+template <int DIR, int G, int M, typename R, u64mask IN, typename TW>
+__device__ __forceinline__ void line_dif_tw(cplx<R> (&v)[M], const Lanes<R, G> &L, TW tw) {
     DftP<M, DIR, R, IN, full_mask(M)>::run(v);
     if constexpr (G > 1) {
-        const int gi = opaque(L.g);
+        tw.begin();
 #pragma unroll
-        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], twN[gi * k]);
+        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], tw(k));
         if constexpr (G >= 8) {
 #pragma unroll
             for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<4>(v[k], L.s4), L.w4);
@@ -119,11 +134,18 @@ __device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
     }
 }
 
+// line_dif with the lane twiddles read from twN[e] = exp(+2 pi i e/N) as
+// they are used (nothing held in registers across calls).
+template <int DIR, int G, int M, typename R, u64mask IN = full_mask(M)>
+__device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
+                                         const cplx<R> *twN) {
+    line_dif_tw<DIR, G, M, R, IN>(v, L, TwLds<R>{twN, L.g});
+}
+
 // DIT transform of a G-lane line, the transpose of line_dif's structure.
 // OUT: the output slots m the caller uses (the others are left undefined).
-template <int DIR, int G, int M, typename R, u64mask OUT = full_mask(M)>
-__device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
-                                         const cplx<R> *twN) {
+template <int DIR, int G, int M, typename R, u64mask OUT, typename TW>
+__device__ __forceinline__ void line_dit_tw(cplx<R> (&v)[M], const Lanes<R, G> &L, TW tw) {
     if constexpr (G > 1) {
 #pragma unroll
         for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
@@ -135,11 +157,17 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
 #pragma unroll
             for (int k = 0; k < M; k++) v[k] = bfly<4>(cmul_dir<DIR>(v[k], L.w4), L.s4);
         }
-        const int gi = opaque(L.g);
+        tw.begin();
 #pragma unroll
-        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], twN[gi * k]);
+        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], tw(k));
     }
     DftP<M, DIR, R, full_mask(M), OUT>::run(v);
+}
+
+template <int DIR, int G, int M, typename R, u64mask OUT = full_mask(M)>
+__device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
+                                         const cplx<R> *twN) {
+    line_dit_tw<DIR, G, M, R, OUT>(v, L, TwLds<R>{twN, L.g});
 }
 
 // ---------------------------------------------------------------------------
@@ -218,6 +246,21 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     {  // rows: inverse x, V(r), forward x
         const Lanes<R, GX> L;
         const int ntask = P * N;
+        // the lane twiddles of every row: in FP64, read once into registers
+        // where they fit (up to 9 per lane; all loads in flight together),
+        // else read from LDS as each transform uses them.  (FP64 blocks are
+        // held to 3 per CU by LDS anyway; in FP32 the 16 extra registers cost
+        // a wave per SIMD and measured 5% slower.)
+        constexpr bool HOIST = sizeof(R) == 8 && MX <= 9;
+        cplx<R> twx[MX];
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int k = 1; k < MX; k++) twx[k] = twN[L.g * k];
+        }
+        const auto tw = [&]() {
+            if constexpr (HOIST) return TwRegs<R, MX>{twx};
+            else return TwLds<R>{twN, L.g};
+        }();
         for (int task = tid / GX; task < ntask; task += T / GX) {
             const int p = task / N, y = task - p * N;
             cplx<R> *row = sub + (p * N + y) * S;
@@ -228,11 +271,11 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
                 const int x = L.g + GX * j;
                 if (has_bit(XB, j)) v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
             }
-            line_dif<+1, GX, MX, R, XB>(v, L, twN);
+            line_dif_tw<+1, GX, MX, R, XB>(v, L, tw);
             const R *vy = vr + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
 #pragma unroll
             for (int k = 0; k < MX; k++) v[k] = scale(v[k], vy[k]);
-            line_dit<-1, GX, MX, R, XB>(v, L, twN);
+            line_dit_tw<-1, GX, MX, R, XB>(v, L, tw);
             // gap columns go to the trash column nx + 1
 #pragma unroll
             for (int m = 0; m < MX; m++) {
