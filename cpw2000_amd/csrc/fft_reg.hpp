@@ -46,12 +46,16 @@ constexpr int bitrev(int v, int bits) {
 }
 
 // Per-lane constants of a G-lane group: position, butterfly signs and the
-// upper-half twiddles W_{2h}^{(g mod 2h) - h} (backward sign) of each stage.
+// upper-half twiddles W_{2h}^{(g mod 2h) - h} (backward sign) of each stage,
+// and for the one-instruction FP32 butterflies (bfly_t) the same twiddles
+// times the sign that the butterfly before them leaves on the upper lanes.
 template <typename R, int G> struct Lanes {
     int g;    // position in the group
     int k2;   // bitrev(g): DIF output / DIT input block index
-    R s1, s2, s4;
+    R s1, s2, s4, t1n;
     cplx<R> w2, w4;
+    cplx<R> w2s, w4s;  // DIF: s2 w2, s4 w4
+    cplx<R> w2d, w4d;  // DIT: s1 w2, s2 w4
     __device__ __forceinline__ Lanes() {
         const int l = (int)(threadIdx.x % G);
         g = (G == 8 && l >= 4) ? 11 - l : l;
@@ -66,6 +70,11 @@ template <typename R, int G> struct Lanes {
         const R h = (R)0.70710678118654752440;
         const int q = g - 4;
         w4 = q == 1 ? mk<R>(h, h) : q == 2 ? mk<R>(0, 1) : q == 3 ? mk<R>(-h, h) : mk<R>(1, 0);
+        t1n = -s1;
+        w2s = mk<R>(s2 * w2.x, s2 * w2.y);
+        w4s = mk<R>(s4 * w4.x, s4 * w4.y);
+        w2d = mk<R>(s1 * w2.x, s1 * w2.y);
+        w4d = mk<R>(s2 * w4.x, s2 * w4.y);
     }
 };
 
@@ -91,6 +100,42 @@ template <int H, typename R> __device__ __forceinline__ cplx<R> bfly(cplx<R> m, 
         return mk<R>(xlane<C>(m.x) + s * m.x, xlane<C>(m.y) + s * m.y);  // partner + s * mine
     }
 }
+// FP32 butterfly in one instruction: mine + t * partner, a v_fmac_f32 whose
+// multiplicand is the partner lane's value (DPP source operand) and whose
+// addend is the lane's own.  t = +-1, so the product is exact and the result
+// is the rounded sum or difference, as in bfly.  On the lane pair (a, b) of a
+// stage, a = lower position, holding sa x and sb y (sa, sb = +-1 known per
+// lane): t_a = sa sb, t_b = -sa sb give a: sa (x + y) and b: -sb (x - y), so
+// with equal signs in, the upper lane's output is negated; the caller folds
+// that sign into the next stage's twiddle (Lanes::w2s, w4s, w2d, w4d) or into
+// the first-stage t of the transform that consumes it (line_dit_tw NEGIN).
+// The s_nop gives the two wait states that a DPP read of a register written by
+// the preceding VALU instruction needs (the compiler does not track hazards
+// inside inline assembly); the five that follow an EXEC write are met by the
+// transform code before every butterfly (loads, DFT_M, twiddles).
+template <int H> __device__ __forceinline__ float fmac_xlane(float mine, float t) {
+    if constexpr (H == 1)
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+            : "+v"(mine) : "v"(t));
+    else if constexpr (H == 2)
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf"
+            : "+v"(mine) : "v"(t));
+    else
+        asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 row_half_mirror row_mask:0xf bank_mask:0xf"
+            : "+v"(mine) : "v"(t));
+    return mine;
+}
+template <int H> __device__ __forceinline__ cplx<float> bfly_t(cplx<float> m, float t) {
+    return mk<float>(fmac_xlane<H>(m.x, t), fmac_xlane<H>(m.y, t));
+}
+// Lines that use bfly_t: FP32 with 4 or 8 lanes, up to length 72.  Its sign
+// constants and asm operands take about 5 more registers per lane; with 2
+// lanes (36 values each) and the longer lines (80-128) the plane kernels then
+// spill, so those keep bfly.
+template <typename R, int G, int M> constexpr bool use_bfly_t() {
+    return sizeof(R) == 4 && G >= 4 && G * M <= 72;
+}
+
 template <int DIR, typename R> __device__ __forceinline__ cplx<R> cmul_dir(cplx<R> a, cplx<R> w) {
     return cmul(a, DIR > 0 ? w : mk<R>(w.x, -w.y));
 }
@@ -114,39 +159,85 @@ template <typename R, int M> struct TwRegs {
     __device__ __forceinline__ cplx<R> operator()(int k) const { return tw[k]; }
 };
 
-template <int DIR, int G, int M, typename R, u64mask IN, typename TW>
+// NEGOUT (lines with bfly_t, else ignored): the last stage also uses bfly_t
+// and leaves the values of the odd positions negated, for a line_dit_tw with
+// NEGIN that consumes them directly (a pointwise real factor in between keeps
+// the pattern).
+template <int DIR, int G, int M, typename R, u64mask IN, bool NEGOUT = false, typename TW>
 __device__ __forceinline__ void line_dif_tw(cplx<R> (&v)[M], const Lanes<R, G> &L, TW tw) {
     DftP<M, DIR, R, IN, full_mask(M)>::run(v);
     if constexpr (G > 1) {
         tw.begin();
 #pragma unroll
         for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], tw(k));
-        if constexpr (G >= 8) {
+        if constexpr (use_bfly_t<R, G, M>()) {
+            // one-instruction butterflies; each upper-lane sign goes into the
+            // stage twiddle that follows
+            if constexpr (G >= 8) {
 #pragma unroll
-            for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<4>(v[k], L.s4), L.w4);
+                for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly_t<4>(v[k], L.s4), L.w4s);
+            }
+            if constexpr (G >= 4) {
+#pragma unroll
+                for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly_t<2>(v[k], L.s2), L.w2s);
+            }
+            if constexpr (NEGOUT) {
+#pragma unroll
+                for (int k = 0; k < M; k++) v[k] = bfly_t<1>(v[k], L.s1);
+            } else {
+#pragma unroll
+                for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
+            }
+        } else {
+            if constexpr (G >= 8) {
+#pragma unroll
+                for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<4>(v[k], L.s4), L.w4);
+            }
+            if constexpr (G >= 4) {
+#pragma unroll
+                for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<2>(v[k], L.s2), L.w2);
+            }
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
         }
-        if constexpr (G >= 4) {
-#pragma unroll
-            for (int k = 0; k < M; k++) v[k] = cmul_dir<DIR>(bfly<2>(v[k], L.s2), L.w2);
-        }
-#pragma unroll
-        for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
     }
 }
 
 // line_dif with the lane twiddles read from twN[e] = exp(+2 pi i e/N) as
 // they are used (nothing held in registers across calls).
-template <int DIR, int G, int M, typename R, u64mask IN = full_mask(M)>
+template <int DIR, int G, int M, typename R, u64mask IN = full_mask(M), bool NEGOUT = false>
 __device__ __forceinline__ void line_dif(cplx<R> (&v)[M], const Lanes<R, G> &L,
                                          const cplx<R> *twN) {
-    line_dif_tw<DIR, G, M, R, IN>(v, L, TwLds<R>{twN, L.g});
+    line_dif_tw<DIR, G, M, R, IN, NEGOUT>(v, L, TwLds<R>{twN, L.g});
 }
 
 // DIT transform of a G-lane line, the transpose of line_dif's structure.
 // OUT: the output slots m the caller uses (the others are left undefined).
-template <int DIR, int G, int M, typename R, u64mask OUT, typename TW>
+// NEGIN (lines with bfly_t, else ignored): the odd positions hold negated
+// values (line_dif_tw NEGOUT).
+template <int DIR, int G, int M, typename R, u64mask OUT, bool NEGIN = false, typename TW>
 __device__ __forceinline__ void line_dit_tw(cplx<R> (&v)[M], const Lanes<R, G> &L, TW tw) {
-    if constexpr (G > 1) {
+    if constexpr (G > 1 && use_bfly_t<R, G, M>()) {
+        // one-instruction butterflies except in the last stage (its output
+        // meets the lane twiddles, of which the first is not multiplied);
+        // with NEGIN, t = -s1 in the first stage gives exact outputs, else its
+        // odd outputs are negated and w2d = s1 w2 restores them; the second
+        // stage's upper-lane sign goes into w4d = s2 w4
+#pragma unroll
+        for (int k = 0; k < M; k++) v[k] = bfly_t<1>(v[k], NEGIN ? L.t1n : L.s1);
+        if constexpr (G >= 8) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly_t<2>(cmul_dir<DIR>(v[k], NEGIN ? L.w2 : L.w2d), L.s2);
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly<4>(cmul_dir<DIR>(v[k], L.w4d), L.s4);
+        } else if constexpr (G >= 4) {
+#pragma unroll
+            for (int k = 0; k < M; k++) v[k] = bfly<2>(cmul_dir<DIR>(v[k], NEGIN ? L.w2 : L.w2d), L.s2);
+        }
+        tw.begin();
+#pragma unroll
+        for (int k = 1; k < M; k++) v[k] = cmul_dir<DIR>(v[k], tw(k));
+    } else if constexpr (G > 1) {
 #pragma unroll
         for (int k = 0; k < M; k++) v[k] = bfly<1>(v[k], L.s1);
         if constexpr (G >= 4) {
@@ -164,10 +255,10 @@ __device__ __forceinline__ void line_dit_tw(cplx<R> (&v)[M], const Lanes<R, G> &
     DftP<M, DIR, R, full_mask(M), OUT>::run(v);
 }
 
-template <int DIR, int G, int M, typename R, u64mask OUT = full_mask(M)>
+template <int DIR, int G, int M, typename R, u64mask OUT = full_mask(M), bool NEGIN = false>
 __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
                                          const cplx<R> *twN) {
-    line_dit_tw<DIR, G, M, R, OUT>(v, L, TwLds<R>{twN, L.g});
+    line_dit_tw<DIR, G, M, R, OUT, NEGIN>(v, L, TwLds<R>{twN, L.g});
 }
 
 // ---------------------------------------------------------------------------
@@ -271,11 +362,12 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
                 const int x = L.g + GX * j;
                 if (has_bit(XB, j)) v[j] = row[x <= kd1 ? x : x >= xhi ? x - ushift : nx];
             }
-            line_dif_tw<+1, GX, MX, R, XB>(v, L, tw);
+            // (with bfly_t the odd positions carry negated values between the two)
+            line_dif_tw<+1, GX, MX, R, XB, true>(v, L, tw);
             const R *vy = vr + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
 #pragma unroll
             for (int k = 0; k < MX; k++) v[k] = scale(v[k], vy[k]);
-            line_dit_tw<-1, GX, MX, R, XB>(v, L, tw);
+            line_dit_tw<-1, GX, MX, R, XB, true>(v, L, tw);
             // gap columns go to the trash column nx + 1
 #pragma unroll
             for (int m = 0; m < MX; m++) {

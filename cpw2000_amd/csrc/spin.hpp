@@ -268,8 +268,11 @@ __global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ c
                 u[j] = ru[o];
                 d[j] = rd[o];
             }
-            line_dif<+1, GX, MX>(u, L, twN);
-            line_dif<+1, GX, MX>(d, L, twN);
+            // (with the one-instruction FP32 butterflies the odd positions of
+            // both components carry negated values until the forward x: the
+            // potential below is linear)
+            line_dif<+1, GX, MX, R, full_mask(MX), true>(u, L, twN);
+            line_dif<+1, GX, MX, R, full_mask(MX), true>(d, L, twN);
             const R *v0 = vsp + ((long)z * g.n[1] + y) * g.id + MX * L.k2;
 #pragma unroll
             for (int k = 0; k < MX; k++) {
@@ -278,8 +281,8 @@ __global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ c
                 u[k] = mk<R>(vu * a.x + vx * b.x + vy * b.y, vu * a.y + vx * b.y - vy * b.x);
                 d[k] = mk<R>(vd * b.x + vx * a.x - vy * a.y, vd * b.y + vx * a.y + vy * a.x);
             }
-            line_dit<-1, GX, MX>(u, L, twN);
-            line_dit<-1, GX, MX>(d, L, twN);
+            line_dit<-1, GX, MX, R, full_mask(MX), true>(u, L, twN);
+            line_dit<-1, GX, MX, R, full_mask(MX), true>(d, L, twN);
 #pragma unroll
             for (int m = 0; m < MX; m++) {
                 const int x = L.g + GX * m;
