@@ -638,7 +638,7 @@ def issue_bound(prec, kernel, variant, n1, avg_ms, rates):
 
 
 ISSUE_KINDS = ("v_fma_f64", "v_add_f64", "v_fma_f32", "v_add_u32", "v_mov_b32_dpp", "v_fma_f64+v_mov_b32_dpp",
-               "v_add_f32", "v_add_f32_dpp", "v_permlane32_swap", "v_permlane16_swap")
+               "v_add_f32", "v_add_f32_dpp", "v_permlane32_swap", "v_permlane16_swap", "v_fmac_f32_dpp")
 
 
 def issue_rates():

@@ -17,7 +17,7 @@ namespace {
 
 constexpr int ITER = 256;
 constexpr int T = 256;
-constexpr int NKIND = 10;
+constexpr int NKIND = 11;
 
 template <int K>
 __global__ __launch_bounds__(T) void k_issue(double *out, long long *cyc, double seed) {
@@ -52,6 +52,9 @@ __global__ __launch_bounds__(T) void k_issue(double *out, long long *cyc, double
                 if constexpr (K == 6) asm volatile("v_add_f32 %0, %0, %1" : "+v"(f[i]) : "v"(fc));
                 if constexpr (K == 7)  // the partner lane's value as the DPP source of the add
                     asm volatile("v_add_f32_dpp %0, %1, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                                 : "+v"(f[i]) : "v"(fc));
+                if constexpr (K == 10)  // FP32 butterfly: the partner's value as the DPP multiplicand
+                    asm volatile("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
                                  : "+v"(f[i]) : "v"(fc));
                 if constexpr (K == 8 || K == 9) {  // gfx950 half-swaps of two registers, 32 / 16 lanes apart
                     const int i2 = (i + 4) & 7;
@@ -140,8 +143,9 @@ extern "C" {
 // v_fma_f32, v_add_u32, v_mov_b32_dpp, the average of an interleaved
 // v_fma_f64 / v_mov_b32_dpp stream, v_add_f32, v_add_f32 with a DPP source
 // operand, v_permlane32_swap_b32 and v_permlane16_swap_b32 (each rewrites two
-// registers); res[10]: shader clock (GHz) during the runs; res[11]: number of
-// SIMDs.  nres >= 12.  Returns 0, or the HIP error code.
+// registers), s_nop 1 + v_fmac_f32 with a DPP source operand (the FP32
+// butterfly); res[11]: shader clock (GHz) during the runs; res[12]: number of
+// SIMDs.  nres >= 13.  Returns 0, or the HIP error code.
 int cpwprobe_issue_rates(double *res, int nres) {
     if (!res || nres < NKIND + 2) return (int)hipErrorInvalidValue;
     int dev = 0, ncu = 0, wall_khz = 0;
@@ -170,6 +174,7 @@ int cpwprobe_issue_rates(double *res, int nres) {
     if (e == hipSuccess) e = measure<7>(nsimd, wall_hz, out, cyc, &res[7], &ghz[7]);
     if (e == hipSuccess) e = measure<8>(nsimd, wall_hz, out, cyc, &res[8], &ghz[8]);
     if (e == hipSuccess) e = measure<9>(nsimd, wall_hz, out, cyc, &res[9], &ghz[9]);
+    if (e == hipSuccess) e = measure<10>(nsimd, wall_hz, out, cyc, &res[10], &ghz[10]);
     double g = 0;
     for (int k = 0; k < NKIND; k++) g += ghz[k] / NKIND;
     res[NKIND] = g;
