@@ -112,7 +112,8 @@ template <int H, typename R> __device__ __forceinline__ cplx<R> bfly(cplx<R> m, 
 // The s_nop gives the two wait states that a DPP read of a register written by
 // the preceding VALU instruction needs (the compiler does not track hazards
 // inside inline assembly); the five that follow an EXEC write are met by the
-// transform code before every butterfly (loads, DFT_M, twiddles).
+// transform code before every butterfly (loads, DFT_M, twiddles).  Both rules
+// are checked on the built library by tests/test_dpp_hazards.py.
 template <int H> __device__ __forceinline__ float fmac_xlane(float mine, float t) {
     if constexpr (H == 1)
         asm("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
