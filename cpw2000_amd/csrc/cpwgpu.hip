@@ -964,6 +964,50 @@ int local_part(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *
     return CPWGPU_OK;
 }
 
+// Local and kinetic terms with the inverse column and plane kernels on stream
+// a and the gather on stream s, after the non-local term the caller enqueued
+// on s (written into hpsi): per band chunk, a runs the inverse columns and the
+// plane stage, s waits for them and gathers (adding to hpsi), and a waits for
+// that gather before the next chunk reuses colbuf.  The caller has ordered a
+// after s's earlier work.
+template <typename R>
+int local_beside(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hpsi, long ldh,
+                 hipStream_t s, hipStream_t a, bool kin_gather, const Work *w) {
+    const FftGeom &g = p->g;
+    if (g.ncol <= 0) return CPWGPU_OK;
+    const size_t cs = sizeof(cplx<R>);
+    const R *vr = sizeof(R) == 8 ? (const R *)p->v64.p : (const R *)p->v32.p;
+    if (!vr) return fail(CPWGPU_ERR_ARG, "potential not set");
+    const size_t per_band = cs * (size_t)g.n[2] * g.ncolp;
+    const size_t cap = w ? w->colbuf_n : (4ull << 30);
+    if (per_band > cap) return fail(CPWGPU_ERR_SIZE, "z-column workspace too small for one band");
+    const int nbc = (int)std::max<size_t>(1, std::min<size_t>(neig, cap / per_band));
+    int st;
+    cplx<R> *colbuf;
+    if (w) {
+        colbuf = (cplx<R> *)w->colbuf;
+    } else {
+        if ((st = p->colbuf.reserve(per_band * nbc))) return st;
+        colbuf = (cplx<R> *)p->colbuf.p;
+    }
+    const int mode = COLF_ACC | (kin_gather ? COLF_KIN : 0);
+    for (int b0 = 0; b0 < neig; b0 += nbc) {
+        const int nb = std::min(nbc, neig - b0);
+        const LocalArgs la{(const double2 *)psi + (long)b0 * ldp, ldp,
+                           (double2 *)hpsi + (long)b0 * ldh, ldh, nb, mode};
+        if (b0 > 0) {  // colbuf is free again after the previous chunk's gather
+            HIPCHK(hipEventRecord(g_fork, s));
+            HIPCHK(hipStreamWaitEvent(a, g_fork, 0));
+        }
+        if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_BWD, a))) return st;
+        if ((st = local_stage<R>(p, la, colbuf, vr, ST_PLANE, a))) return st;
+        HIPCHK(hipEventRecord(g_join, a));
+        HIPCHK(hipStreamWaitEvent(s, g_join, 0));
+        if ((st = local_stage<R>(p, la, colbuf, vr, ST_COL_FWD, s))) return st;
+    }
+    return CPWGPU_OK;
+}
+
 // w: caller workspace (see Work), or null for the plan's buffers.  With a
 // workspace the second stream is used only when it already exists.
 template <typename R>
@@ -991,6 +1035,18 @@ int apply_t(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *hps
         if ((st = local_part<R>(p, neig, psi, ldp, hpsi, ldh, false, nullptr, s, kg, w, true))) return st;
         HIPCHK(hipStreamWaitEvent(s, g_join, 0));
         return nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 1, !kg, s, w, NL_BACK);
+    }
+    if (nl && p->sched_var[sizeof(R) == 4 ? 1 : 0] == 2) {
+        // fork the other way round: the inverse column and plane kernels on
+        // the second stream, the non-local term on the caller's stream (written
+        // into hpsi), then the gather there once the plane stage is done.  The
+        // plane stage usually ends before the back-projection, so the gather's
+        // wait finds its event complete.
+        if ((st = ensure_aux())) return st;
+        HIPCHK(hipEventRecord(g_fork, s));
+        HIPCHK(hipStreamWaitEvent(g_aux, g_fork, 0));
+        if ((st = nonlocal<R>(p, scalar_proj(p), neig, psi, ldp, hpsi, ldh, 0, !kg, s, w))) return st;
+        return local_beside<R>(p, neig, psi, ldp, hpsi, ldh, s, g_aux, kg, w);
     }
     if (nl) {
         // fork: non-local term on the second stream, written into hpsi
@@ -2562,7 +2618,7 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
     const TuneKey key{prec, g.n[0], g.n[1], g.n[2], g.nx, g.ncol, p->nanl, neig};
     int pv = -1, jv = -1, bv = -1, sk = -1, kv = -1, cv = -1, sv = -1;
     if (tune_cache_lookup(key, &pv, &jv, &bv, &sk, &kv, &cv, &sv)) {
-        p->sched_var[ix] = sv == 0 || sv == 1 ? sv : -1;
+        p->sched_var[ix] = sv >= 0 && sv <= 2 ? sv : -1;
         bool ok;
         if (prec == CPWGPU_PREC_FP32) plane_var_resolve<float>(p, pv, &ok);
         else plane_var_resolve<double>(p, pv, &ok);
@@ -2639,17 +2695,19 @@ int tune_plan(cpwgpu_plan_s *p, int neig, const double *psi, long ldp, double *h
         }
     }
     p->kin_var[ix] = best_kv;
-    // stream schedule (two-stream applies only); with the back-projection
-    // after the gather it no longer shares the GPU: choose its kernel again
+    // stream schedule (two-stream applies only); schedules 1 and 2 change
+    // what the back-projection runs beside: choose its kernel again
     int best_sv = 0;
     if (p->nanl > 0 && g_overlap) {
         p->sched_var[ix] = 0;
         if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &best))) return st;
-        p->sched_var[ix] = 1;
-        for (int bv = 0; bv < NBACK_VAR; bv++) {
-            p->back_var[ix] = bv;
-            if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
-            if (t < best) { best = t; best_sv = 1; best_bv = bv; }
+        for (int sv = 1; sv <= 2; sv++) {
+            p->sched_var[ix] = sv;
+            for (int bv = 0; bv < NBACK_VAR; bv++) {
+                p->back_var[ix] = bv;
+                if ((st = time_apply(p, neig, psi, ldp, hpsi, ldh, prec, s, 5, &t))) return st;
+                if (t < best) { best = t; best_sv = sv; best_bv = bv; }
+            }
         }
         p->back_var[ix] = best_bv;
     }
@@ -3255,7 +3313,7 @@ int cpwgpu_plan_set_variant(cpwgpu_plan_t plan, int prec, int what, int value) {
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_SCHED) {
-        if (value > 1) return fail(CPWGPU_ERR_MODE, "stream schedule %d", value);
+        if (value > 2) return fail(CPWGPU_ERR_MODE, "stream schedule %d", value);
         plan->sched_var[ix] = value < 0 ? -1 : value;
         return CPWGPU_OK;
     }
@@ -3307,7 +3365,7 @@ int cpwgpu_plan_get_variant(cpwgpu_plan_t plan, int prec, int what, int *value) 
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_SCHED) {
-        *value = plan->sched_var[ix] == 1 ? 1 : 0;
+        *value = plan->sched_var[ix] == 1 || plan->sched_var[ix] == 2 ? plan->sched_var[ix] : 0;
         return CPWGPU_OK;
     }
     if (what == CPWGPU_TUNE_COL) {

@@ -438,7 +438,9 @@ class Plan:
         0 LDS two-pass, 1 register lines where built, -1 default), TUNE_COL
         (z-column kernels: 0 LDS two-pass, 1 register lines where built) or
         TUNE_SCHED (stream schedule: 0 the whole non-local term beside the
-        local one, 1 projection beside it and back-projection after it)."""
+        local one, 1 projection beside it and back-projection after it, 2 the
+        non-local term on the caller's stream with the column and plane
+        kernels beside it)."""
         _check(lib().cpwgpu_plan_set_variant(self._h, int(prec), int(what), int(value)))
 
     def get_variant(self, prec, what):

@@ -243,7 +243,9 @@ int cpwgpu_plan_info(cpwgpu_plan_t plan, int *out);
  *                       the local term to it; 1 = the projection and D scaling
  *                       beside the local term, the gather writing H psi, then
  *                       the back-projection adding to it on the caller's
- *                       stream; -1 = default (0) */
+ *                       stream; 2 = the inverse column and plane kernels on
+ *                       the second stream, the whole non-local term on the
+ *                       caller's stream, then the gather there; -1 = default (0) */
 #define CPWGPU_TUNE_PLANE 0
 #define CPWGPU_TUNE_SPLITK 1
 #define CPWGPU_TUNE_PROJ 2

@@ -93,12 +93,12 @@ def test_plane_variants_match_reference(torch_dev, hk, name):
         for v in vs:
             plan.set_variant(prec, hk.TUNE_PLANE, v)
             assert plan.get_variant(prec, hk.TUNE_PLANE) == v
-            for jv in (0, 1):
-                plan.set_variant(prec, hk.TUNE_PROJ, jv)
-                plan.set_variant(prec, hk.TUNE_BACK, min(jv, 3))
-                plan.set_variant(prec, hk.TUNE_KIN, jv)  # kinetic term in the epilogue / the gather
+            for jv in (0, 1, 2):
+                plan.set_variant(prec, hk.TUNE_PROJ, jv % 2)
+                plan.set_variant(prec, hk.TUNE_BACK, jv % 2)
+                plan.set_variant(prec, hk.TUNE_KIN, jv % 2)  # kinetic term in the epilogue / the gather
                 plan.set_variant(prec, hk.TUNE_COL, cvs[jv % len(cvs)])
-                plan.set_variant(prec, hk.TUNE_SCHED, 1 - jv)
+                plan.set_variant(prec, hk.TUNE_SCHED, (1, 0, 2)[jv])
                 h = _apply(torch, plan, pt, prec).cpu().numpy().T
                 assert rel(h, href) < tol, (name, prec, v, jv, rel(h, href))
         plan.set_variant(prec, hk.TUNE_COL, -1)
@@ -124,7 +124,7 @@ def test_variants_agree_on_all_bands_cfg1(torch_dev, hk):
             h = _apply(torch, plan, pt, prec)
             assert float(torch.linalg.norm(h - h0) / torch.linalg.norm(h0)) < tol, (prec, "back", bv)
         plan.set_variant(prec, hk.TUNE_BACK, -1)
-        for sv in (1, 0):  # stream schedules, with both kinetic-term placements
+        for sv in (1, 2, 0):  # stream schedules, with both kinetic-term placements
             plan.set_variant(prec, hk.TUNE_SCHED, sv)
             assert plan.get_variant(prec, hk.TUNE_SCHED) == sv
             for kv in (1, 0):
@@ -156,7 +156,7 @@ def test_bad_variants_are_rejected(hk):
     plan = hk.Plan.from_inputs(d)
     for what, value in ((hk.TUNE_PLANE, hk.NPLANE_VAR), (hk.TUNE_SPLITK, 10), (hk.TUNE_SPLITK, 5000),
                         (hk.TUNE_PROJ, 6), (hk.TUNE_BACK, 5), (hk.TUNE_KIN, 2), (hk.TUNE_SPIN, 2),
-                        (hk.TUNE_COL, 2), (hk.TUNE_SCHED, 2), (8, 0)):
+                        (hk.TUNE_COL, 2), (hk.TUNE_SCHED, 3), (8, 0)):
         with pytest.raises(hk.CpwGpuError):
             plan.set_variant(hk.PREC_FP32, what, value)
     with pytest.raises(hk.CpwGpuError):
@@ -191,7 +191,7 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
         assert choice["plane"] in _variants(hk, plan, prec)
         assert choice["proj"] in range(6) and choice["back"] in range(5)
         assert choice["splitk"] in (128, 256, 512, 1024)
-        assert choice["kin"] in (0, 1) and choice["col"] in (0, 1) and choice["sched"] in (0, 1)
+        assert choice["kin"] in (0, 1) and choice["col"] in (0, 1) and choice["sched"] in (0, 1, 2)
         plan.set_variant(prec, hk.TUNE_COL, choice["col"])
         plan.set_variant(prec, hk.TUNE_SCHED, choice["sched"])
         plan.set_variant(prec, hk.TUNE_PLANE, choice["plane"])
