@@ -829,10 +829,12 @@ def main():
     for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
         _, cal[prec] = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 3, 1, None)
     work = {p: kernel_work(info, nanl, p) for p in ("fp64", "fp32")}
-    # the local stage on one stream (no projector GEMMs beside it)
+    # the local stage on one stream (no projector GEMMs beside it), and every
+    # kernel's duration with nothing beside it (picks the dominant kernel)
     hk.set_overlap(False)
-    local = {}
+    local, alone = {}, {}
     for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
+        _, alone[prec] = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 3, 1, None)
         _, per = time_path(torch, hk, plan, psi, h64 if pid == 0 else h32, pid, 5, 1,
                            ["k_col_bwd", "k_plane", "k_col_fwd"])
         local[prec] = local_roofline(info, nb, prec, per)
@@ -840,7 +842,9 @@ def main():
     log(f"bench: local stage alone {local}")
 
     def dominant(prec):
-        return max((k for k in cal[prec] if k in work[prec]), key=lambda k: cal[prec][k])
+        # the kernel with the most work: the longest one alone (under overlap
+        # the kernels beside each other stretch by turns, with the schedule)
+        return max((k for k in alone[prec] if k in work[prec]), key=lambda k: alone[prec][k])
 
     res = {}
     for prec, pid in (("fp64", hk.PREC_FP64), ("fp32", hk.PREC_FP32)):
@@ -884,6 +888,7 @@ def main():
         "local_roofline": local,
         "tuned": tuned,
         "kernel_ms": kernels,
+        "kernel_ms_alone": {p: {k: round(v, 5) for k, v in alone[p].items()} for p in alone},
         "kernel_roofline": {p: [roofline_entry(k, work[p], cal[p][k], nb, p)
                                 for k in work[p] if k in cal[p]] for p in cal},
     }
