@@ -140,8 +140,11 @@ struct Prof {
     std::vector<hipEvent_t> pool;
     hipEvent_t get() {
         if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        // timing only (read after the whole stream has finished): no
+        // system-scope fence, so recording them does not write back and
+        // invalidate the caches in the middle of the work they time
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         return e;
     }
 } g_prof;
