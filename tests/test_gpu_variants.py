@@ -217,3 +217,25 @@ def test_tuner_choice_cache_and_output(torch_dev, hk, tmp_path, monkeypatch):
     assert len([l for l in cache.read_text().split("\n") if l.strip()]) == 2
     plan.close()
     plan2.close()
+
+
+def test_noalloc_under_every_stream_schedule(torch_dev, hk):
+    """The allocation-free entry point runs apply_t on its fixed workspace in
+    band chunks; with the second stream present each stream schedule (0, 1, 2)
+    must give the same result as the allocating entry point with that
+    schedule, on the 64-atom cell (several chunks)."""
+    torch = torch_dev
+    d = synth.make_config(1)
+    plan = hk.Plan.from_inputs(d)
+    mtxd = d["mtxd"]
+    pt = torch.from_numpy(np.ascontiguousarray(d["psi"][:mtxd].T)).to("cuda")
+    for prec, tol in ((hk.PREC_FP64, 1e-12), (hk.PREC_FP32, 2e-6)):
+        for sv in (0, 1, 2):
+            plan.set_variant(prec, hk.TUNE_SCHED, sv)
+            ref = _apply(torch, plan, pt, prec)  # also creates the second stream
+            got = torch.full_like(pt, float("nan"))
+            plan.apply_noalloc(pt, got, prec)
+            torch.cuda.synchronize()
+            assert float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref)) < tol, (prec, sv)
+        plan.set_variant(prec, hk.TUNE_SCHED, -1)
+    plan.close()
