@@ -21,14 +21,19 @@ def torch_dev():
     return torch
 
 
+@pytest.mark.parametrize("sched", [0, 1, 2])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_graph_replays_equal_eager(torch_dev, orc, prec):
+def test_graph_replays_equal_eager(torch_dev, orc, prec, sched):
+    """Under each stream schedule (the fork and join of the two streams are
+    captured into the graph as well)."""
     torch = torch_dev
     from cpw2000_amd import hk_psi
 
     ds = [synth.make_config(0, kpt=k) for k in KPTS]
     assert len({d["mtxd"] for d in ds}) > 1
     plans = [hk_psi.Plan.from_inputs(d) for d in ds]
+    for p in plans:
+        p.set_variant(prec, hk_psi.TUNE_SCHED, sched)
     psis = [torch.from_numpy(np.ascontiguousarray(d["psi"][: d["mtxd"]].T)).to("cuda") for d in ds]
     eager = []
     for p, x in zip(plans, psis):
