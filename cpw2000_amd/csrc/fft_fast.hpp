@@ -98,12 +98,35 @@ __global__ __launch_bounds__(FT) void k_col_bwd_fast(
     if (!xcd_band_tile(blockIdx.x, ntiles, nb, band, tile)) return;
     const int c0 = tile * CT;
     const int nc = min(CT, g.ncol - c0);
+    const double2 *pb = psi + (long)band * ldp;
+    const int e0 = col_ptr[c0], e1 = col_ptr[c0 + nc];
+    // The first CB_PRE sphere entries of each thread (index, then value: two
+    // dependent global round trips) are loaded before the twiddles and the
+    // clearing of the lines, which they overlap; each tile has at least one
+    // entry, so a clamped entry index is always valid.
+    constexpr int CB_PRE = 4;
+    int pcz[CB_PRE];
+    double2 pv[CB_PRE];
+    {
+        int pi[CB_PRE];
+#pragma unroll
+        for (int q = 0; q < CB_PRE; q++) {
+            const int e = e0 + (int)threadIdx.x + q * FT;
+            const int ec = e < e1 ? e : e0;
+            pcz[q] = ent_cz[ec];
+            pi[q] = ent_i[ec];
+        }
+#pragma unroll
+        for (int q = 0; q < CB_PRE; q++) pv[q] = pb[pi[q]];
+    }
     load_twiddles(tw, tw3, N3);
     for (int t = threadIdx.x; t < CT * LS; t += FT) lines[t] = mk<R>(0, 0);
     __syncthreads();
-    const double2 *pb = psi + (long)band * ldp;
-    const int e0 = col_ptr[c0], e1 = col_ptr[c0 + nc];
-    for (int e = e0 + threadIdx.x; e < e1; e += FT) {
+#pragma unroll
+    for (int q = 0; q < CB_PRE; q++)
+        if (e0 + (int)threadIdx.x + q * FT < e1)
+            lines[((pcz[q] >> 12) - c0) * LS + (pcz[q] & 0xfff)] = mk<R>((R)pv[q].x, (R)pv[q].y);
+    for (int e = e0 + (int)threadIdx.x + CB_PRE * FT; e < e1; e += FT) {
         const int cz = ent_cz[e];
         const double2 p = pb[ent_i[e]];
         lines[((cz >> 12) - c0) * LS + (cz & 0xfff)] = mk<R>((R)p.x, (R)p.y);
@@ -162,10 +185,23 @@ __global__ __launch_bounds__(FT) void k_col_fwd_fast(
             ph[q].y += ek * p.y;
         }
     }
+    // the tile's columns into the lines: every load of a thread issued before
+    // the first LDS store, so the round trips overlap (loads at clamped,
+    // valid positions; only the positions that exist are stored)
     const cplx<R> *ib = colbuf + (long)band * N3 * g.ncolp + c0;
-    for (int t = threadIdx.x; t < N3 * CT; t += FT) {
+    constexpr int NPT = (N3 * CT + FT - 1) / FT;
+    cplx<R> cv[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; u++) {
+        const int t = min((int)threadIdx.x + u * FT, N3 * CT - 1);
         const int z = t / CT, cc = t - z * CT;
-        if (cc < nc) lines[cc * LS + z] = ib[(long)z * g.ncolp + cc];
+        cv[u] = ib[(long)z * g.ncolp + min(cc, nc - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < NPT; u++) {
+        const int t = (int)threadIdx.x + u * FT;
+        const int z = t / CT, cc = t - z * CT;
+        if (t < N3 * CT && cc < nc) lines[cc * LS + z] = cv[u];
     }
     __syncthreads();
     line_fft<R, N3, -1>(lines, nc, LS, 1, tw);
