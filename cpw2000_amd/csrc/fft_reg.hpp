@@ -303,13 +303,33 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     const int nx = g.nx, kd1 = g.kd[0], ushift = N - nx, xhi = N - kd1;
     const int tid = threadIdx.x;
 
+    // FP32: the first GPRE z-column values of each thread (first plane) are
+    // loaded before the twiddles and the clearing of the sub-plane, which they
+    // overlap (clamped: the kernel runs only with g.ncol >= 1).  (In FP64 the
+    // compiler keeps these arrays in scratch memory, so FP64 loads as it goes.)
+    constexpr int GPRE = 4;
+    constexpr int NPRE = sizeof(R) == 4 ? GPRE : 0;
+    int puy[GPRE];
+    cplx<R> pv[GPRE];
+    if constexpr (NPRE > 0) {
+        const cplx<R> *cb = colbuf + ((long)b0 * n3 + z) * g.ncolp;
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) {
+            const int c = min(tid + u * T, g.ncol - 1);
+            puy[u] = col_uy[c];
+            pv[u] = cb[c];
+        }
+    }
     load_twiddles(twN, twg, N);
     for (int q = tid; q < P * N * S; q += T) sub[q] = mk<R>(0, 0);
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NPRE; u++)
+        if (tid + u * T < g.ncol) sub[(puy[u] >> 16) * S + (puy[u] & 0xffff)] = pv[u];
     for (int p = 0; p < np; p++) {
         const cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
         cplx<R> *sp = sub + p * N * S;
-        for (int c = tid; c < g.ncol; c += T) {
+        for (int c = tid + (p == 0 ? NPRE * T : 0); c < g.ncol; c += T) {
             const int uy = col_uy[c];
             sp[(uy >> 16) * S + (uy & 0xffff)] = cb[c];
         }
@@ -398,12 +418,18 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
         }
     }
     __syncthreads();
+    // scatter back, GPRE columns per thread at a time: their map entries are
+    // loaded together before the LDS reads and stores
     for (int p = 0; p < np; p++) {
         cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
         const cplx<R> *sp = sub + p * N * S;
-        for (int c = tid; c < g.ncol; c += T) {
-            const int uy = col_uy[c];
-            cb[c] = sp[(uy >> 16) * S + (uy & 0xffff)];
+        for (int c0 = tid; c0 < g.ncol; c0 += GPRE * T) {
+            int uy[GPRE];
+#pragma unroll
+            for (int u = 0; u < GPRE; u++) uy[u] = col_uy[min(c0 + u * T, g.ncol - 1)];
+#pragma unroll
+            for (int u = 0; u < GPRE; u++)
+                if (c0 + u * T < g.ncol) cb[c0 + u * T] = sp[(uy[u] >> 16) * S + (uy[u] & 0xffff)];
         }
     }
 }
