@@ -303,29 +303,28 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     const int nx = g.nx, kd1 = g.kd[0], ushift = N - nx, xhi = N - kd1;
     const int tid = threadIdx.x;
 
-    // FP32: the first GPRE z-column values of each thread (first plane) are
-    // loaded before the twiddles and the clearing of the sub-plane, which they
-    // overlap (clamped: the kernel runs only with g.ncol >= 1).  (In FP64 the
-    // compiler keeps these arrays in scratch memory, so FP64 loads as it goes.)
-    constexpr int GPRE = 4;
-    constexpr int NPRE = sizeof(R) == 4 ? GPRE : 0;
-    int puy[GPRE];
-    cplx<R> pv[GPRE];
-    if constexpr (NPRE > 0) {
-        const cplx<R> *cb = colbuf + ((long)b0 * n3 + z) * g.ncolp;
-#pragma unroll
-        for (int u = 0; u < NPRE; u++) {
-            const int c = min(tid + u * T, g.ncol - 1);
-            puy[u] = col_uy[c];
-            pv[u] = cb[c];
-        }
-    }
+    // the first GPRE z-column values of each thread (first plane) are loaded
+    // before the twiddles and the clearing of the sub-plane, which they
+    // overlap (clamped: the kernel runs only with g.ncol >= 1); four named
+    // values, not an array, so that FP64 keeps them in registers too
+    constexpr int GPRE = 4, NPRE = GPRE;
+    struct Pre { int uy; cplx<R> v; };
+    const cplx<R> *cb0 = colbuf + ((long)b0 * n3 + z) * g.ncolp;
+    const auto pre = [&](int u) {
+        const int c = min(tid + u * T, g.ncol - 1);
+        return Pre{col_uy[c], cb0[c]};
+    };
+    const Pre q0 = pre(0), q1 = pre(1), q2 = pre(2), q3 = pre(3);
     load_twiddles(twN, twg, N);
     for (int q = tid; q < P * N * S; q += T) sub[q] = mk<R>(0, 0);
     __syncthreads();
-#pragma unroll
-    for (int u = 0; u < NPRE; u++)
-        if (tid + u * T < g.ncol) sub[(puy[u] >> 16) * S + (puy[u] & 0xffff)] = pv[u];
+    const auto put = [&](int u, const Pre &q) {
+        if (tid + u * T < g.ncol) sub[(q.uy >> 16) * S + (q.uy & 0xffff)] = q.v;
+    };
+    put(0, q0);
+    put(1, q1);
+    put(2, q2);
+    put(3, q3);
     for (int p = 0; p < np; p++) {
         const cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
         cplx<R> *sp = sub + p * N * S;
