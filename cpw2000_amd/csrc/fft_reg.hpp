@@ -262,6 +262,46 @@ __device__ __forceinline__ void line_dit(cplx<R> (&v)[M], const Lanes<R, G> &L,
     line_dit_tw<DIR, G, M, R, OUT, NEGIN>(v, L, TwLds<R>{twN, L.g});
 }
 
+// z-column values of one plane between colbuf (cb: ncol values) and the
+// sub-plane sp (row stride S; col_uy[c] = y << 16 | u), from column c_begin,
+// four columns of each thread at a time with their global loads issued
+// together (one round trip per four columns, not one per column).  Clamped
+// loads: ncol >= 1.
+template <typename R, int T>
+__device__ __forceinline__ void gather_plane(cplx<R> *sp, const cplx<R> *__restrict__ cb,
+                                             const int *__restrict__ col_uy, int ncol, int S, int c_begin) {
+    struct Pre { int uy; cplx<R> v; };
+    for (int c0 = c_begin + (int)threadIdx.x; c0 < ncol; c0 += 4 * T) {
+        const auto ld = [&](int u) {
+            const int c = min(c0 + u * T, ncol - 1);
+            return Pre{col_uy[c], cb[c]};
+        };
+        const Pre q0 = ld(0), q1 = ld(1), q2 = ld(2), q3 = ld(3);
+        const auto put = [&](int u, const Pre &q) {
+            if (c0 + u * T < ncol) sp[(q.uy >> 16) * S + (q.uy & 0xffff)] = q.v;
+        };
+        put(0, q0);
+        put(1, q1);
+        put(2, q2);
+        put(3, q3);
+    }
+}
+template <typename R, int T>
+__device__ __forceinline__ void scatter_plane(cplx<R> *__restrict__ cb, const cplx<R> *sp,
+                                              const int *__restrict__ col_uy, int ncol, int S) {
+    for (int c0 = threadIdx.x; c0 < ncol; c0 += 4 * T) {
+        const int u0 = col_uy[min(c0, ncol - 1)], u1 = col_uy[min(c0 + T, ncol - 1)];
+        const int u2 = col_uy[min(c0 + 2 * T, ncol - 1)], u3 = col_uy[min(c0 + 3 * T, ncol - 1)];
+        const auto put = [&](int u, int uy) {
+            if (c0 + u * T < ncol) cb[c0 + u * T] = sp[(uy >> 16) * S + (uy & 0xffff)];
+        };
+        put(0, u0);
+        put(1, u1);
+        put(2, u2);
+        put(3, u3);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // plane stage, register variant: a block takes P planes (bands b0..b0+P-1 at
 // one z) with T threads and a register budget for W waves per SIMD.  LDS:
@@ -325,14 +365,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
     put(1, q1);
     put(2, q2);
     put(3, q3);
-    for (int p = 0; p < np; p++) {
-        const cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
-        cplx<R> *sp = sub + p * N * S;
-        for (int c = tid + (p == 0 ? NPRE * T : 0); c < g.ncol; c += T) {
-            const int uy = col_uy[c];
-            sp[(uy >> 16) * S + (uy & 0xffff)] = cb[c];
-        }
-    }
+    for (int p = 0; p < np; p++)
+        gather_plane<R, T>(sub + p * N * S, colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp, col_uy, g.ncol, S,
+                           p == 0 ? NPRE * T : 0);
     __syncthreads();
 
     if constexpr (YW) {
@@ -417,20 +452,9 @@ __global__ __launch_bounds__(T, W) void k_plane_reg(cplx<R> *__restrict__ colbuf
         }
     }
     __syncthreads();
-    // scatter back, GPRE columns per thread at a time: their map entries are
-    // loaded together before the LDS reads and stores
-    for (int p = 0; p < np; p++) {
-        cplx<R> *cb = colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp;
-        const cplx<R> *sp = sub + p * N * S;
-        for (int c0 = tid; c0 < g.ncol; c0 += GPRE * T) {
-            int uy[GPRE];
-#pragma unroll
-            for (int u = 0; u < GPRE; u++) uy[u] = col_uy[min(c0 + u * T, g.ncol - 1)];
-#pragma unroll
-            for (int u = 0; u < GPRE; u++)
-                if (c0 + u * T < g.ncol) cb[c0 + u * T] = sp[(uy[u] >> 16) * S + (uy[u] & 0xffff)];
-        }
-    }
+    // scatter back, the map entries of four columns per thread loaded together
+    for (int p = 0; p < np; p++)
+        scatter_plane<R, T>(colbuf + ((long)(b0 + p) * n3 + z) * g.ncolp, sub + p * N * S, col_uy, g.ncol, S);
 }
 
 // LDS bytes of k_plane_reg

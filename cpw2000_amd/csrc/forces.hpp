@@ -179,11 +179,7 @@ __global__ __launch_bounds__(T, W) void k_density_reg(const cplx<double> *__rest
     for (int b = b0; b < b1; b++) {
         for (int q = tid; q < N * S; q += T) sub[q] = mk<R>(0, 0);
         __syncthreads();
-        const cplx<R> *cb = colbuf + ((long)b * n3 + z) * g.ncolp;
-        for (int c = tid; c < g.ncol; c += T) {
-            const int uy = col_uy[c];
-            sub[(uy >> 16) * S + (uy & 0xffff)] = cb[c];
-        }
+        gather_plane<R, T>(sub, colbuf + ((long)b * n3 + z) * g.ncolp, col_uy, g.ncol, S, 0);
         __syncthreads();
         // inverse y on the nx box lines
         for (int u = tid / GY; u < nx; u += T / GY) {

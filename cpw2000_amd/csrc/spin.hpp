@@ -233,12 +233,8 @@ __global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ c
     cplx<R> *cb[2];
     cb[0] = colbuf + ((long)pair * n3 + z) * g.ncolp;
     cb[1] = colbuf + ((long)(pair + npair) * n3 + z) * g.ncolp;
-    for (int c = tid; c < g.ncol; c += T) {
-        const int uy = col_uy[c];
-        const int o = (uy >> 16) * S + (uy & 0xffff);
-        sub[o] = cb[0][c];
-        sub[N * S + o] = cb[1][c];
-    }
+    gather_plane<R, T>(sub, cb[0], col_uy, g.ncol, S, 0);
+    gather_plane<R, T>(sub + N * S, cb[1], col_uy, g.ncol, S, 0);
     __syncthreads();
 
     {  // inverse y on the 2 * nx box lines
@@ -308,12 +304,8 @@ __global__ __launch_bounds__(T, W) void k_plane_spin_reg(cplx<R> *__restrict__ c
         }
     }
     __syncthreads();
-    for (int c = tid; c < g.ncol; c += T) {
-        const int uy = col_uy[c];
-        const int o = (uy >> 16) * S + (uy & 0xffff);
-        cb[0][c] = sub[o];
-        cb[1][c] = sub[N * S + o];
-    }
+    scatter_plane<R, T>(cb[0], sub, col_uy, g.ncol, S);
+    scatter_plane<R, T>(cb[1], sub + N * S, col_uy, g.ncol, S);
 }
 
 }  // namespace cpwgpu
