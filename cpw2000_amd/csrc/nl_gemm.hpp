@@ -629,27 +629,29 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_back(const cplx<R> *__restrict
         __syncthreads();
     }
     if (i < mtxd) {
+        // the kinetic term in FP64 (coalesced rows of psi) and the term already
+        // in hpsi: the loads of four outputs issued together (at clamped rows:
+        // a row beyond neig is loaded and not used), then their stores
+        const double ek = kpsi ? ekpg[i] : 0.0;
 #pragma unroll
-        for (int ni = 0; ni < 4; ni++)
+        for (int ni = 0; ni < 4; ni++) {
+            double2 kp[4], ho[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long nc = min(n0 + ni * 16 + MF<R>::row(lg, r), neig - 1);
+                kp[r] = kpsi ? kpsi[nc * ldk + i] : make_double2(0.0, 0.0);
+                ho[r] = beta ? hpsi[nc * ldh + i] : make_double2(0.0, 0.0);
+            }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int n = n0 + ni * 16 + MF<R>::row(lg, r);
                 if (n < neig) {
                     const R p1 = P1[ni][r], p2 = P2[ni][r], p3 = P3[ni][r];
-                    double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
-                    if (kpsi) {  // kinetic term in FP64 (coalesced rows of psi)
-                        const double2 p = kpsi[(long)n * ldk + i];
-                        h.x += ekpg[i] * p.x;
-                        h.y += ekpg[i] * p.y;
-                    }
-                    if (beta) {
-                        const double2 o = hpsi[(long)n * ldh + i];
-                        h.x += o.x;
-                        h.y += o.y;
-                    }
-                    hpsi[(long)n * ldh + i] = h;
+                    hpsi[(long)n * ldh + i] = make_double2((double)(p1 - p2) + ek * kp[r].x + ho[r].x,
+                                                           (double)(p3 - p1 - p2) + ek * kp[r].y + ho[r].y);
                 }
             }
+        }
     }
 }
 
@@ -789,27 +791,27 @@ __global__ __launch_bounds__(GEMM_THREADS, (BackDirectCfg<R, V>::W)) void k_back
     for (int q = 0; q < IT; q++) {
         if (!iok[q]) continue;
         const int i = i0 + q * 16 + lr;
+        // as in k_back: the loads of four outputs issued together, then the stores
+        const double ek = kpsi ? ekpg[i] : 0.0;
 #pragma unroll
-        for (int t = 0; t < NT; t++)
+        for (int t = 0; t < NT; t++) {
+            double2 kp[4], ho[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long nc = min(n0 + t * 16 + MF<R>::row(lg, r), neig - 1);
+                kp[r] = kpsi ? kpsi[nc * ldk + i] : make_double2(0.0, 0.0);
+                ho[r] = beta ? hpsi[nc * ldh + i] : make_double2(0.0, 0.0);
+            }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int n = n0 + t * 16 + MF<R>::row(lg, r);
                 if (n < neig) {
                     const R p1 = P1[t][q][r], p2 = P2[t][q][r], p3 = P3[t][q][r];
-                    double2 h = make_double2((double)(p1 - p2), (double)(p3 - p1 - p2));
-                    if (kpsi) {  // kinetic term in FP64 (coalesced rows of psi)
-                        const double2 p = kpsi[(long)n * ldk + i];
-                        h.x += ekpg[i] * p.x;
-                        h.y += ekpg[i] * p.y;
-                    }
-                    if (beta) {
-                        const double2 o = hpsi[(long)n * ldh + i];
-                        h.x += o.x;
-                        h.y += o.y;
-                    }
-                    hpsi[(long)n * ldh + i] = h;
+                    hpsi[(long)n * ldh + i] = make_double2((double)(p1 - p2) + ek * kp[r].x + ho[r].x,
+                                                           (double)(p3 - p1 - p2) + ek * kp[r].y + ho[r].y);
                 }
             }
+        }
     }
 }
 
