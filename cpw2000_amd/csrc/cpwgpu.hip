@@ -6,6 +6,7 @@
 // band-batched FFT kernels (fft_local.hpp) and the non-local term
 // (Src/cpw_lib/hk_psi_nl_c16.f90) two MFMA GEMMs (nl_gemm.hpp).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -165,6 +166,24 @@ struct ProfScope {
         if (b && hipEventRecord(b, s) == hipSuccess) g_prof.recs.push_back({k, a, b});
     }
 };
+
+// Launches kernel f on stream s.  With the measurement hook on for kernel id
+// k, the start and stop events travel with the dispatch itself (the kernel's
+// own timestamps, hipExtLaunchKernelGGL) instead of as marker packets
+// recorded around it, which would idle the stream between the kernels.
+template <typename... Args, typename F = void (*)(Args...)>
+void launch_timed(int k, F f, dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args) {
+    if (g_prof.on && (g_prof.mask >> k & 1u)) {
+        hipEvent_t a = g_prof.get(), b = a ? g_prof.get() : nullptr;
+        if (a && b) {
+            hipExtLaunchKernelGGL(f, grid, block, shmem, s, a, b, 0u, args...);
+            g_prof.recs.push_back({k, a, b});
+            return;
+        }
+        if (a) g_prof.pool.push_back(a);
+    }
+    hipLaunchKernelGGL(f, grid, block, shmem, s, args...);
+}
 
 constexpr int COL_CT = 16;   // z-columns per block of the column kernels
 constexpr int KMAX = 8;      // butterflies per thread per FFT stage
@@ -515,10 +534,9 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
-        ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane_reg<R, N, GX, GY, P, T, W, YW, PR>), dim3(((a.nb + P - 1) / P) * g.n[2]),
-                           dim3(T), dyn, s, colbuf, (const int *)p->col_uy.p, vr,
-                           (const double2 *)p->tw[0].p, g, a.nb, S);
+        launch_timed(CPWGPU_K_PLANE, k_plane_reg<R, N, GX, GY, P, T, W, YW, PR>,
+                     dim3(((a.nb + P - 1) / P) * g.n[2]), dim3(T), (unsigned)dyn, s, colbuf,
+                     (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p, g, a.nb, S);
         HIPCHK(hipGetLastError());
         return CPWGPU_OK;
     }
