@@ -172,11 +172,11 @@ struct ProfScope {
 // own timestamps, hipExtLaunchKernelGGL) instead of as marker packets
 // recorded around it, which would idle the stream between the kernels.
 template <typename... Args, typename F = void (*)(Args...)>
-void launch_timed(int k, F f, dim3 grid, dim3 block, unsigned shmem, hipStream_t s, Args... args) {
+void launch_timed(int k, F f, dim3 grid, dim3 block, size_t shmem, hipStream_t s, Args... args) {
     if (g_prof.on && (g_prof.mask >> k & 1u)) {
         hipEvent_t a = g_prof.get(), b = a ? g_prof.get() : nullptr;
         if (a && b) {
-            hipExtLaunchKernelGGL(f, grid, block, shmem, s, a, b, 0u, args...);
+            hipExtLaunchKernelGGL(f, grid, block, (std::uint32_t)shmem, s, a, b, 0u, args...);
             g_prof.recs.push_back({k, a, b});
             return;
         }
@@ -411,13 +411,11 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
     const int ntiles = (g.ncol + CT - 1) / CT;
     const dim3 grid(8 * ((a.nb + 7) / 8) * ntiles);
     if (stage == ST_COL_BWD) {
-        ProfScope ps_(CPWGPU_K_COL_BWD, s);
-        hipLaunchKernelGGL((k_col_bwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, colbuf,
+        launch_timed(CPWGPU_K_COL_BWD, (k_col_bwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, colbuf,
                            (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
                            (const int *)p->ent_cz.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb);
     } else if (stage == ST_COL_FWD) {
-        ProfScope ps_(CPWGPU_K_COL_FWD, s);
-        hipLaunchKernelGGL((k_col_fwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, a.hpsi, a.ldh,
+        launch_timed(CPWGPU_K_COL_FWD, (k_col_fwd_fast<R, N>), grid, dim3(FT), 0, s, a.psi, a.ldp, a.hpsi, a.ldh,
                            (const cplx<R> *)colbuf, (const int *)p->col_ptr.p,
                            (const int *)p->ent_i.p, (const int *)p->ent_cz.p,
                            (const double *)p->ekpg.p, (const double2 *)p->tw[2].p, g, ntiles, a.nb,
@@ -435,8 +433,7 @@ int launch_fast(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
                                        (int)(160 * 1024 - stat)));
             attr = true;
         }
-        ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane_fast<R, N, N>), dim3(g.n[2] * a.nb), dim3(FT), dyn, s, colbuf,
+        launch_timed(CPWGPU_K_PLANE, (k_plane_fast<R, N, N>), dim3(g.n[2] * a.nb), dim3(FT), dyn, s, colbuf,
                            (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p,
                            (const double2 *)p->tw[1].p, g);
     }
@@ -535,7 +532,7 @@ int launch_plane_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, cons
             attr = true;
         }
         launch_timed(CPWGPU_K_PLANE, k_plane_reg<R, N, GX, GY, P, T, W, YW, PR>,
-                     dim3(((a.nb + P - 1) / P) * g.n[2]), dim3(T), (unsigned)dyn, s, colbuf,
+                     dim3(((a.nb + P - 1) / P) * g.n[2]), dim3(T), dyn, s, colbuf,
                      (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p, g, a.nb, S);
         HIPCHK(hipGetLastError());
         return CPWGPU_OK;
@@ -555,8 +552,7 @@ int launch_plane_wave(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, con
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat)));
         attr = true;
     }
-    ProfScope ps_(CPWGPU_K_PLANE, s);
-    hipLaunchKernelGGL((k_plane_wave<R, N, N, T, 2>), dim3(g.n[2] * a.nb), dim3(T), dyn, s, colbuf,
+    launch_timed(CPWGPU_K_PLANE, (k_plane_wave<R, N, N, T, 2>), dim3(g.n[2] * a.nb), dim3(T), dyn, s, colbuf,
                        (const int *)p->col_uy.p, vr, (const double2 *)p->tw[0].p, (const double2 *)p->tw[1].p, g);
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
@@ -585,21 +581,19 @@ int launch_col_reg(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, int st
         const double2 *tw3 = (const double2 *)p->tw[2].p;
         const int *zm = (const int *)p->zmap.p;
         if (stage == ST_COL_BWD) {
-            ProfScope ps_(CPWGPU_K_COL_BWD, s);
             if (pr)
-                hipLaunchKernelGGL((k_col_bwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
+                launch_timed(CPWGPU_K_COL_BWD, (k_col_bwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
                                    zm, tw3, g, ntiles, a.nb);
             else
-                hipLaunchKernelGGL((k_col_bwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
+                launch_timed(CPWGPU_K_COL_BWD, (k_col_bwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, colbuf,
                                    zm, tw3, g, ntiles, a.nb);
         } else {
-            ProfScope ps_(CPWGPU_K_COL_FWD, s);
             if (pr)
-                hipLaunchKernelGGL((k_col_fwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
+                launch_timed(CPWGPU_K_COL_FWD, (k_col_fwd_reg<R, N, G, true>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
                                    a.ldh, (const cplx<R> *)colbuf, zm, (const double *)p->ekpg.p, tw3, g, ntiles,
                                    a.nb, p->inv_n, a.mode);
             else
-                hipLaunchKernelGGL((k_col_fwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
+                launch_timed(CPWGPU_K_COL_FWD, (k_col_fwd_reg<R, N, G, false>), grid, dim3(COLR_T), 0, s, a.psi, a.ldp, a.hpsi,
                                    a.ldh, (const cplx<R> *)colbuf, zm, (const double *)p->ekpg.p, tw3, g, ntiles,
                                    a.nb, p->inv_n, a.mode);
         }
@@ -720,17 +714,14 @@ int local_stage(cpwgpu_plan_s *p, const LocalArgs &a, cplx<R> *colbuf, const R *
     const double2 *tw1 = (const double2 *)p->tw[0].p, *tw2 = (const double2 *)p->tw[1].p,
                   *tw3 = (const double2 *)p->tw[2].p;
     if (stage == ST_COL_BWD) {
-        ProfScope ps_(CPWGPU_K_COL_BWD, s);
-        hipLaunchKernelGGL((k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
+        launch_timed(CPWGPU_K_COL_BWD, (k_col_bwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
                            a.psi, a.ldp, colbuf, (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
                            (const int *)p->ent_cz.p, tw3, g, ntiles);
     } else if (stage == ST_PLANE) {
-        ProfScope ps_(CPWGPU_K_PLANE, s);
-        hipLaunchKernelGGL((k_plane<R, KMAX>), dim3(g.n[2] * a.nb), dim3(PLANE_THREADS), lplane, s,
+        launch_timed(CPWGPU_K_PLANE, (k_plane<R, KMAX>), dim3(g.n[2] * a.nb), dim3(PLANE_THREADS), lplane, s,
                            colbuf, (const int *)p->col_uy.p, vr, tw1, tw2, g, rb);
     } else {
-        ProfScope ps_(CPWGPU_K_COL_FWD, s);
-        hipLaunchKernelGGL((k_col_fwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
+        launch_timed(CPWGPU_K_COL_FWD, (k_col_fwd<R, COL_CT, KMAX>), dim3(ntiles * a.nb), dim3(COL_THREADS), lcol, s,
                            a.psi, a.ldp, a.hpsi, a.ldh, (const cplx<R> *)colbuf,
                            (const int *)p->col_ptr.p, (const int *)p->ent_i.p,
                            (const int *)p->ent_cz.p, (const double *)p->ekpg.p, tw3, g, ntiles,
@@ -868,32 +859,30 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
         dhd = (cplx<R> *)p->dhd.p;
     }
     if (phase != NL_BACK) {
-        ProfScope ps_(CPWGPU_K_PROJ, s);
         const dim3 grid(tm, tn, nsl);
         const double2 *ps = (const double2 *)psi;
         cplx<R> *pt = part;
         if (pv == 1)
-            hipLaunchKernelGGL((k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
+            launch_timed(CPWGPU_K_PROJ, (k_proj_direct<R, 1>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
         else if (pv == 2)
-            hipLaunchKernelGGL((k_proj_direct<R, 2>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
+            launch_timed(CPWGPU_K_PROJ, (k_proj_direct<R, 2>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
         else if (pv == 3)
-            hipLaunchKernelGGL((k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
+            launch_timed(CPWGPU_K_PROJ, (k_proj_direct<R, 3>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
         else if (pv == 4)
-            hipLaunchKernelGGL((k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice);
+            launch_timed(CPWGPU_K_PROJ, (k_proj_direct<R, 4>), grid, dim3(GEMM_THREADS), 0, s, anl, lda, ps, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
         else if (pv == NPROJ_DIRECT + 1)
-            hipLaunchKernelGGL((k_proj<R, proj_half_wm<R>(), proj_half_wn<R>()>), dim3(tm, tn, nsl),
+            launch_timed(CPWGPU_K_PROJ, (k_proj<R, proj_half_wm<R>(), proj_half_wn<R>()>), dim3(tm, tn, nsl),
                                dim3(GEMM_THREADS), 0, s, anl, lda, (const double2 *)psi, ldp,
-                               pt, rows, nanl, neig, kslice, nullptr);
+                               pt, rows, nanl, neig, kslice, nullptr, 0);
         else
-            hipLaunchKernelGGL(k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
-                               (const double2 *)psi, ldp, pt, rows, nanl, neig, kslice, nullptr);
+            launch_timed(CPWGPU_K_PROJ, k_proj<R>, dim3(tm, tn, nsl), dim3(GEMM_THREADS), 0, s, anl, lda,
+                               (const double2 *)psi, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
     }
     if (phase != NL_BACK) {
         ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
         launch_proj_reduce<R>((const cplx<R> *)part, pv_.xnl, dhd, nanl, neig, nsl, s);
     }
     if (phase != NL_PROJ) {
-        ProfScope ps_(CPWGPU_K_BACK, s);
         const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_VAR - 1 ? p->back_var[ix] : 0;
         const cplx<R> *dh = dhd;
         double2 *hp = (double2 *)hpsi;
@@ -902,7 +891,7 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
         const double *ek = kin ? (const double *)p->ekpg.p : nullptr;
 #define CPW_BD(V)                                                                                  \
     if (bv == V)                                                                                   \
-        hipLaunchKernelGGL((k_back_direct<R, V>),                                                  \
+        launch_timed(CPWGPU_K_BACK, (k_back_direct<R, V>),                                                  \
                            dim3((rows + back_bi<R, V>() - 1) / back_bi<R, V>(),                  \
                                 (neig + back_bn<R, V>() - 1) / back_bn<R, V>()),                   \
                            dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta, kp,   \
@@ -910,11 +899,11 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
         CPW_BD(1) CPW_BD(2) CPW_BD(3)
 #undef CPW_BD
         if (bv == 0)
-            hipLaunchKernelGGL(k_back<R>, dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
+            launch_timed(CPWGPU_K_BACK, k_back<R>, dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
                                dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta,
                                kp, ldp, ek);
         else if (bv == NBACK_DIRECT + 1)
-            hipLaunchKernelGGL((k_back<R, GemmCfg<R>::KC / 2>),
+            launch_timed(CPWGPU_K_BACK, (k_back<R, GemmCfg<R>::KC / 2>),
                                dim3((rows + BP_BI - 1) / BP_BI, (neig + BP_BN - 1) / BP_BN),
                                dim3(GEMM_THREADS), 0, s, anl, lda, dh, hp, ldh, rows, nanl, neig, beta,
                                kp, ldp, ek);
@@ -1387,12 +1376,11 @@ int launch_plane_spin_reg(cpwgpu_plan_s *p, cplx<R> *colbuf, const R *vr, long v
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = true;
         }
-        ProfScope ps_(CPWGPU_K_PLANE, s);
         if (cs)
-            hipLaunchKernelGGL((k_plane_spin_reg<R, N, GX, GY, T, W, true>), dim3(g.n[2] * npair), dim3(T), dyn, s,
+            launch_timed(CPWGPU_K_PLANE, (k_plane_spin_reg<R, N, GX, GY, T, W, true>), dim3(g.n[2] * npair), dim3(T), dyn, s,
                                colbuf, (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p, g, npair, S);
         else
-            hipLaunchKernelGGL((k_plane_spin_reg<R, N, GX, GY, T, W, false>), dim3(g.n[2] * npair), dim3(T), dyn, s,
+            launch_timed(CPWGPU_K_PLANE, (k_plane_spin_reg<R, N, GX, GY, T, W, false>), dim3(g.n[2] * npair), dim3(T), dyn, s,
                                colbuf, (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p, g, npair, S);
         HIPCHK(hipGetLastError());
         return CPWGPU_OK;
@@ -1414,8 +1402,7 @@ int launch_plane_spin(cpwgpu_plan_s *p, cplx<R> *colbuf, const R *vr, long vstri
                                    (int)(160 * 1024 - stat)));
         attr = true;
     }
-    ProfScope ps_(CPWGPU_K_PLANE, s);
-    hipLaunchKernelGGL((k_plane_spin<R, N, N>), dim3(g.n[2] * npair), dim3(FT), dyn, s, colbuf,
+    launch_timed(CPWGPU_K_PLANE, (k_plane_spin<R, N, N>), dim3(g.n[2] * npair), dim3(FT), dyn, s, colbuf,
                        (const int *)p->col_uy.p, vr, vstride, (const double2 *)p->tw[0].p,
                        (const double2 *)p->tw[1].p, g, npair);
     HIPCHK(hipGetLastError());
