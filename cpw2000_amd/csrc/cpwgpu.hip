@@ -150,23 +150,6 @@ struct Prof {
     }
 } g_prof;
 
-struct ProfScope {
-    int k;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    ProfScope(int k_, hipStream_t s_) : k(k_), s(s_) {
-        if (g_prof.on && (g_prof.mask >> k & 1u)) {
-            a = g_prof.get();
-            if (a && hipEventRecord(a, s) != hipSuccess) a = nullptr;
-        }
-    }
-    ~ProfScope() {
-        if (!a) return;
-        hipEvent_t b = g_prof.get();
-        if (b && hipEventRecord(b, s) == hipSuccess) g_prof.recs.push_back({k, a, b});
-    }
-};
-
 // Launches kernel f on stream s.  With the measurement hook on for kernel id
 // k, the start and stop events travel with the dispatch itself (the kernel's
 // own timestamps, hipExtLaunchKernelGGL) instead of as marker packets
@@ -879,8 +862,11 @@ int nonlocal(cpwgpu_plan_s *p, const ProjView &pv_, int neig, const double *psi,
                                (const double2 *)psi, ldp, pt, rows, nanl, neig, kslice, nullptr, 0);
     }
     if (phase != NL_BACK) {
-        ProfScope ps_(CPWGPU_K_PROJ_REDUCE, s);
-        launch_proj_reduce<R>((const cplx<R> *)part, pv_.xnl, dhd, nanl, neig, nsl, s);
+        // launch_proj_reduce's launch, with the measurement hook
+        const long tot = (long)nanl * neig;
+        if (tot > 0)
+            launch_timed(CPWGPU_K_PROJ_REDUCE, k_proj_reduce<R>, dim3((unsigned)((tot + RED_OUT - 1) / RED_OUT)),
+                         dim3(RED_OUT * RED_GRP), 0, s, (const cplx<R> *)part, pv_.xnl, dhd, nanl, neig, nsl, 0);
     }
     if (phase != NL_PROJ) {
         const int bv = p->back_var[ix] >= 1 && p->back_var[ix] <= NBACK_VAR - 1 ? p->back_var[ix] : 0;
@@ -1262,10 +1248,9 @@ int kb_build(const KbSetup &s, const double *gw_dev, int mtxd, double2 *anl, lon
         (st = upload(g_kb_atoms, s.atoms.data(), s.atoms.size())))
         return st;
     const dim3 grid((unsigned)((mtxd + KB_THREADS - 1) / KB_THREADS), (unsigned)s.atoms.size());
-    ProfScope ps(CPWGPU_K_PROJ_KB, g_stream);
-    hipLaunchKernelGGL(k_proj_kb, grid, dim3(KB_THREADS), 0, g_stream, gw_dev, mtxd, s.cell,
-                       (const KbAtom *)g_kb_atoms.p, (const KbType *)g_kb_types.p, (const double *)g_kb_tab.p,
-                       s.vstride, anl, lda);
+    launch_timed(CPWGPU_K_PROJ_KB, k_proj_kb, grid, dim3(KB_THREADS), 0, g_stream, gw_dev, mtxd, s.cell,
+                 (const KbAtom *)g_kb_atoms.p, (const KbType *)g_kb_types.p, (const double *)g_kb_tab.p,
+                 s.vstride, anl, lda);
     HIPCHK(hipGetLastError());
     return CPWGPU_OK;
 }
