@@ -748,10 +748,24 @@ def main():
     ap.add_argument("--issue-rates", action="store_true",
                     help="only measure the issue cost of the FFT kernels' VALU instruction kinds "
                          "(cycles per wave instruction per SIMD) and the shader clock; prints JSON")
+    ap.add_argument("--set", action="append", default=[], metavar="PREC.KIND=V",
+                    help="after tuning, override one choice, e.g. fp32.proj=2 (kinds: plane, "
+                         "proj, back, splitk, kin, col, sched): for profiling a configuration "
+                         "the tuner may or may not pick")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed loops, one plain 512 MiB device copy (a known-size "
                          "dispatch that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    overrides = []
+    for item in args.set:
+        try:
+            lhs, val = item.split("=")
+            prec, kind = lhs.split(".")
+            overrides.append((prec, kind, int(val)))
+        except ValueError:
+            ap.error(f"--set {item!r}: expected PREC.KIND=V")
+        if prec not in ("fp32", "fp64") or kind not in ("plane", "proj", "back", "splitk", "kin", "col", "sched"):
+            ap.error(f"--set {item!r}: PREC is fp32 or fp64, KIND one of plane proj back splitk kin col sched")
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only (--gpus 1): H|psi> of independent "
@@ -799,6 +813,13 @@ def main():
     tuned = {}
     for prec, pid, h in (("fp64", hk.PREC_FP64, h64), ("fp32", hk.PREC_FP32, h32)):
         tuned[prec] = plan.tune(psi, h, pid)
+    for prec, kind, val in overrides:
+        pid = {"fp64": hk.PREC_FP64, "fp32": hk.PREC_FP32}[prec]
+        what = {"plane": hk.TUNE_PLANE, "proj": hk.TUNE_PROJ, "back": hk.TUNE_BACK, "splitk": hk.TUNE_SPLITK,
+                "kin": hk.TUNE_KIN, "col": hk.TUNE_COL, "sched": hk.TUNE_SCHED}[kind]
+        plan.set_variant(pid, what, val)
+        tuned[prec][kind] = plan.get_variant(pid, what)
+        log(f"bench: {prec} {kind} set to {tuned[prec][kind]}")
     if args.only == "ditsp":
         print(json.dumps({"ditsp": bench_ditsp(torch, hk, plan, d, nb, np), "tuned": tuned}), flush=True)
         plan.close()
