@@ -17,9 +17,9 @@ FP64 application of H to the FP32-mixed Ritz vectors.
 
 Before timing, the library's autotuner picks the plane-stage FFT kernel and the
 split-K shape of the projection GEMM for each precision on this workload
-(reported under "tuned").  Per-kernel durations come from HIP events recorded
-around each launch on the launch stream inside the timed loops (library
-measurement hook).
+(reported under "tuned").  Per-kernel durations come from HIP start/stop
+events carried in each launch's dispatch on the launch stream inside the timed
+loops (library measurement hook): the kernels' own durations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
 """
@@ -523,7 +523,7 @@ def bench_projectors(torch, hk, synth, np, d, steps, cfg=1,
     construction of Src/omp/proj_nl_kb_c16.f90) for the cell of ``d``
     (default the 64-atom Si cell: 256 s and p projectors on 23,847 plane
     waves) from interpolation tables of 4,242 points per (species, l).  Kernel
-    time from HIP events around each launch; per-call time includes the table
+    time from HIP events in each launch's dispatch; per-call time includes the table
     upload and the synchronisation.  Algorithmic HBM bytes per launch: the
     projectors written once (16 B per element) plus g of each plane wave read
     once (24 B)."""
@@ -663,8 +663,8 @@ def issue_rates():
 
 def variant_sweep(torch, hk, plan, psi, info, nanl, nb, reps):
     """Standalone (one stream) duration of every plane-stage, projection and
-    back-projection kernel built, both precisions, by HIP events around each
-    launch; then the 512 MiB calibration copy.  For profiling these kernels one
+    back-projection kernel built, both precisions, by HIP events in each
+    launch's dispatch; then the 512 MiB calibration copy.  For profiling these kernels one
     by one."""
     hk.set_overlap(False)
     h = torch.zeros_like(psi)

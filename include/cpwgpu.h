@@ -434,8 +434,9 @@ int cpwgpu_plan_forces(cpwgpu_plan_t plan, int neig, const double *psi_dev, long
 #define CPWGPU_K_BACK 5          /* hpsi += beta D<beta|psi> GEMM                  */
 #define CPWGPU_K_PROJ_KB 6       /* KB projector construction                      */
 #define CPWGPU_NKERNEL 7
-/* When on, later applies record a HIP event pair around each launch of the
- * kernels selected by mask (bit k = kernel k) on the launch stream. */
+/* When on, each later launch of the kernels selected by mask (bit k = kernel
+ * k) carries a HIP start/stop event pair in its dispatch on the launch stream
+ * (the kernel's own start and end). */
 int cpwgpu_profile(int on, unsigned mask);
 /* Waits for the recorded events; ms[k], launches[k] (k < n) = summed
  * durations and launch counts per kernel since the previous read. */
