@@ -101,3 +101,16 @@ def test_projector_traffic_record_is_the_write_stream():
     algo = 16 * INFO["mtxd"] * NANL + 24 * INFO["mtxd"]
     assert algo <= e["bytes"] <= 1.1 * algo
     assert abs(e["bytes"] - (e["fetch_kib"] * rec["fetch_scale"] + e["write_kib"] * rec["write_scale"]) * 1024) < 2048
+
+
+@pytest.mark.parametrize("item,msg", [("fp32.prj=2", "KIND one of"), ("fp16.proj=2", "PREC is fp32 or fp64"),
+                                      ("fp32.proj", "expected PREC.KIND=V"), ("fp32.proj=x", "expected PREC.KIND=V")])
+def test_set_override_is_checked_before_any_gpu_work(item, msg):
+    """bench.py --set rejects a malformed override at argument parsing (exit
+    status 2), before it imports the library or touches a GPU."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--set", item],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and msg in r.stderr, r.stderr[-400:]
